@@ -1,0 +1,158 @@
+/*
+ * born_hip.h -- C ABI of the MI355X (gfx950) Born propagator library
+ * (sc17-born_amd/libborn_hip.so).
+ *
+ * This is the drop-in replacement for the reference's CUDA backend interface
+ * base/gpu/gpu_funcs_3d.h:4-37 and for the baseProp virtuals it serves
+ * (base/base_prop.h:10-68).  Every entry point of the first group names the
+ * reference function it replaces; arguments keep the reference's meaning and
+ * order.  Differences, all deliberate:
+ *   - a context handle replaces the reference's file-scope device globals
+ *     (gpu_finite_3d.h:37-99), so several propagators may coexist;
+ *   - every call returns a status (BORN_OK or a negative BORN_ERR_*) and
+ *     born_last_error() explains it, where the reference calls exit(-1)
+ *     (gpu_funcs_3d.cu:58-70);
+ *   - host-pointer calls follow the CPU backend's semantics (base/cpu_prop.cc),
+ *     which is the parity target: the forward writes the traces into the buffer
+ *     registered by born_transfer_receiver_func, source propagation returns the
+ *     final two levels in p0/p1, and the adjoint accumulates into img.
+ *
+ * Layout: every 3-D array is [nz][ny][nx] float32 with x fastest
+ * (base/cpu_prop.cc:229-231).  Host traces are [nrec][nt_big]
+ * (base/rtm_zero_op_3d.cc:134); device-resident traces are time-major
+ * [nt_big][nrec].
+ *
+ * Threading: one context per host thread; calls on one context are
+ * serialised on the context's stream.
+ */
+#ifndef BORN_HIP_H
+#define BORN_HIP_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BORN_OK 0
+#define BORN_ERR_ARG (-1)         /* invalid argument or size mismatch */
+#define BORN_ERR_STATE (-2)       /* call order violated (missing transfer) */
+#define BORN_ERR_HIP (-3)         /* HIP runtime error */
+#define BORN_ERR_NOMEM (-4)       /* device allocation failed */
+#define BORN_ERR_UNSUPPORTED (-5) /* geometry the kernels do not handle */
+
+typedef struct born_ctx born_ctx;
+
+/* ---- reference interface (gpu_funcs_3d.h / baseProp) -------------------- */
+
+/* setup_cuda(int) (gpu_funcs_3d.h:7, gpuProp ctor gpu_prop.cc:6-12):
+ * binds the context to one HIP device and creates its stream. */
+int born_create(born_ctx **ctx, int device);
+int born_destroy(born_ctx *ctx);
+const char *born_last_error(const born_ctx *ctx);
+
+/* create_gpu_space (gpu_funcs_3d.h:35) / baseProp::createSpace
+ * (base_prop.h:68; cpu_prop.cc:298-332).  Like the CPU backend, bc_a/bc_b/bc_y
+ * are accepted and ignored: the sponge is 40 cells with bcB = .0005. */
+int born_create_space(born_ctx *ctx, float d1, float d2, float d3, float bc_a,
+                      float bc_b, float bc_y, int nx, int ny, int nz);
+
+/* set_ntblock (gpu_funcs_3d.h:14) / baseProp::setNtblock (base_prop.h:13).
+ * Accepted and ignored: trace blocking is not needed (traces stay in HBM). */
+int born_set_ntblock(born_ctx *ctx, int nb);
+
+/* transfer_sinc_table_s / _d (gpu_funcs_3d.h:18,21) / baseProp
+ * transferSincTableS / D (base_prop.h:55,17).  table is [jt][nsinc]
+ * row-major (the flattening of gpu_prop.cc:15-24); nsinc must be 8. */
+int born_transfer_sinc_table_s(born_ctx *ctx, int nsinc, int jts,
+                               const float *table);
+int born_transfer_sinc_table_d(born_ctx *ctx, int nsinc, int jtd,
+                               const float *table);
+
+/* transfer_source_func (gpu_funcs_3d.h:23) / transferSourceFunc
+ * (base_prop.h:22).  vals is [npts][nt_big]; at most 16 points, all inside the
+ * stencil interior. */
+int born_transfer_source_func(born_ctx *ctx, int npts, int nt_big,
+                              const int *locs, const float *vals);
+
+/* transfer_vel_func1 / 2 (gpu_funcs_3d.h:31,33) / transferVelFunc1/2
+ * (base_prop.h:27-34): v^2 dt^2 over the domain.  vel1 drives source
+ * propagation and its reversal, vel2 the Born fields. */
+int born_transfer_vel_func1(born_ctx *ctx, int nx, int ny, int nz,
+                            const float *v2dt2);
+int born_transfer_vel_func2(born_ctx *ctx, int nx, int ny, int nz,
+                            const float *v2dt2);
+
+/* transfer_receiver_func (gpu_funcs_3d.h:26) / transferReceiverFunc
+ * (base_prop.h:42).  Registers the receiver cells and, like the CPU backend
+ * (cpu_prop.cc:281-288), the caller's trace buffer [rec_nx*rec_ny][nt_big]:
+ * born_rtm_forward writes into it and born_rtm_adjoint reads it.  Receivers
+ * must lie inside the stencil interior. */
+int born_transfer_receiver_func(born_ctx *ctx, int rec_nx, int rec_ny,
+                                int nt_big, const int *locs, float *traces);
+
+/* rtm_forward (gpu_funcs_3d.h:15) / rtmForward (base_prop.h:48;
+ * cpu_prop.cc:22-54): Born modelling of the domain image img (n1*n2*n3),
+ * accumulated into the registered trace buffer.  As in the CPU backend the
+ * rec argument is not used. */
+int born_rtm_forward(born_ctx *ctx, int n1, int n2, int n3, int jt,
+                     const float *img, float *rec, int npts, int nt,
+                     int nt_big, int rec_nx, int rec_ny);
+
+/* source_prop (gpu_funcs_3d.h:9) / sourceProp (base_prop.h:35;
+ * cpu_prop.cc:106-129): nt+1 undamped steps with vel1 from the state in
+ * p0/p1; the final two levels are returned in p0/p1 exactly as the CPU backend
+ * leaves them.  damp, get_last and npts are ignored, as there. */
+int born_source_prop(born_ctx *ctx, int nx, int ny, int nz, int damp,
+                     int get_last, float *p0, float *p1, int jts, int npts,
+                     int nt);
+
+/* rtm_adjoint (gpu_funcs_3d.h:12) / rtmAdjoint (base_prop.h:60;
+ * cpu_prop.cc:56-96): reverse-time migration of the registered traces with
+ * the background held in p0/p1; accumulates into img (n1*n2*n3).  p0 and p1
+ * are used as scratch (modified), as in the CPU backend. */
+int born_rtm_adjoint(born_ctx *ctx, int n1, int n2, int n3, int jtd, float *p0,
+                     float *p1, float *img, int npts_s, int nt);
+
+/* ---- extensions (no reference counterpart) ------------------------------ */
+
+/* Sponge width and decay (the reference hard-codes 40 and .0005). */
+int born_set_sponge(born_ctx *ctx, int width, float bcB);
+
+/* Exact adjoint of born_rtm_forward (scatter 0) or of the linearised damped
+ * wave equation in V = v^2 dt^2 (scatter 1): img (n1*n2*n3) += F^T rec, with
+ * rec [nrec][nt_big].  The background is rebuilt from HBM checkpoints. */
+int born_adjoint_exact(born_ctx *ctx, int n1, int n2, int n3, int jt,
+                       const float *rec, float *img, int nt, int scatter);
+
+/* Born forward with a choice of scatter term (0: the reference's 1e-6 s m;
+ * 1: dV * B * L(s)); traces accumulate into rec [nrec][nt_big]. */
+int born_forward_scatter(born_ctx *ctx, int n1, int n2, int n3, int jt,
+                         const float *img, float *rec, int nt, int scatter);
+
+/* Device-resident entry points: all pointers are device memory on the
+ * context's device; traces are time-major [nt_big][nrec].  Work is queued on
+ * the context stream (born_get_stream) and is not synchronised. */
+int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter,
+                     const float *model, float *traces, int keep_background);
+int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
+                           const float *traces, float *img);
+int born_dev_source_prop(born_ctx *ctx, int jts, int nt, float *p0, float *p1);
+int born_dev_rtm_adjoint(born_ctx *ctx, int nt, float *p0, float *p1,
+                         float *img, const float *traces);
+
+/* HBM budget (bytes) for the background history of the exact adjoint;
+ * 0 = all free device memory less a reserve.  The number of checkpoints is
+ * the smallest that fits (1 segment = the whole history kept). */
+int born_set_checkpoint_budget(born_ctx *ctx, size_t bytes);
+/* segments and recomputed steps of the last exact adjoint */
+int born_last_checkpoint_plan(const born_ctx *ctx, int *segments,
+                              int *recomputed_steps);
+/* z planes per block of the step kernels (0 = default) */
+int born_set_zchunk(born_ctx *ctx, int zchunk);
+
+void *born_get_stream(const born_ctx *ctx);
+int born_synchronize(born_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

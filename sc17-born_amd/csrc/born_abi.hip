@@ -1,0 +1,935 @@
+// C ABI of the Born propagator (include/born_hip.h): context, transfers and
+// the time loops of the reference backend (base/cpu_prop.cc), each step one
+// launch of a fused kernel from born_kernels.hip.
+//
+// Exact adjoint.  The Born forward's scattered field obeys
+//   r_{k+1} = A r_k - B^2 r_{k-1} + f_k,   A = B (V L + 2),  d_k = P_k r_{k+1}
+// (B sponge, V = v^2 dt^2, L the symmetric interior stencil).  Its adjoint
+// state lambda obeys lambda_{k+1} = A^T lambda_{k+2} - B^2 lambda_{k+3} +
+// P_k^T w_k with A^T = (L V + 2) B.  The change of variables nu = V B lambda
+// turns that into nu_{k+1} = B (V L nu_{k+2} + 2 nu_{k+2} - B nu_{k+3}) +
+// V B P_k^T w_k: the forward kernel again, with V B-weighted receiver
+// injection.  The model gradient sum_k sigma_k lambda_{k+1} is accumulated as
+// sum_k sigma_k nu_{k+1} and divided by V B once at the end.  sigma_k (the
+// scatter operand of forward step k) is needed in reverse order; it is
+// rebuilt segment by segment from checkpoints of the background, with the
+// number of segments the smallest that fits the HBM budget (one segment:
+// the whole history is kept and nothing is recomputed).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "born_hip.h"
+#include "born_launch.h"
+
+using born::Geometry;
+
+struct born_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  bool have_space = false;
+  int nx = 0, ny = 0, nz = 0;
+  long long n = 0;
+  float coeffs[13] = {};
+  int nbound = 40;
+  float bcB = .0005f;
+  float *d_bx = nullptr, *d_by = nullptr, *d_bz = nullptr;
+  int zchunk = 0;
+
+  int nsinc_s = 0, jts = 0, nsinc_d = 0, jtd = 0;
+  float *d_tab_s = nullptr, *d_tab_d = nullptr;
+
+  int npts_s = 0, nt_src = 0;
+  float *d_src = nullptr;
+  born::SrcPts pts{};
+
+  float *d_vel1 = nullptr, *d_vel2 = nullptr;
+
+  int rec_nx = 0, rec_ny = 0, nrec = 0, nt_rec = 0;
+  float *h_rec = nullptr;
+  int *d_plane_slot = nullptr, *d_rmap = nullptr, *d_rnext = nullptr;
+  int nslots = 0;
+
+  int jt_state = 1;
+
+  float *d_w[4] = {nullptr, nullptr, nullptr, nullptr};
+  float *d_img = nullptr, *d_acc = nullptr, *d_p0 = nullptr, *d_p1 = nullptr;
+  float *d_tr = nullptr, *d_tr2 = nullptr;
+  size_t tr_cap = 0;
+  float *d_amp = nullptr;
+  size_t amp_cap = 0;
+
+  size_t budget = 0;
+  float *d_seg = nullptr;
+  size_t seg_cap = 0;  // fields
+  float *d_ckpt = nullptr;
+  size_t ckpt_cap = 0;  // fields
+  long long bg_version = 1;
+  long long hist_version = -1;
+  int hist_scatter = -1, hist_jt = -1, hist_nt = -1;
+  int last_segments = 0, last_recomputed = 0;
+};
+
+namespace {
+
+int fail(born_ctx *c, int code, const std::string &m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define HIPCHK(expr)                                                       \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return fail(ctx, BORN_ERR_HIP,                                       \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define CHK(expr)                   \
+  do {                              \
+    int r_ = (expr);                \
+    if (r_ != BORN_OK) return r_;   \
+  } while (0)
+
+template <class T>
+int dev_alloc(born_ctx *ctx, T **p, size_t count) {
+  if (*p) {
+    // work queued on the stream may still read the old buffer
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) return BORN_OK;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(ctx, BORN_ERR_NOMEM,
+                "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                    " bytes failed: " + hipGetErrorString(e));
+  }
+  return BORN_OK;
+}
+
+// grow-only device buffer of floats
+int ensure(born_ctx *ctx, float **p, size_t *cap, size_t need) {
+  if (*cap >= need && *p) return BORN_OK;
+  *cap = 0;
+  CHK(dev_alloc(ctx, p, need));
+  *cap = need;
+  return BORN_OK;
+}
+
+int set_device(born_ctx *ctx) {
+  HIPCHK(hipSetDevice(ctx->device));
+  return BORN_OK;
+}
+
+Geometry geometry(const born_ctx *c) {
+  Geometry g;
+  g.nx = c->nx;
+  g.ny = c->ny;
+  g.nz = c->nz;
+  g.zchunk = c->zchunk;
+  // reference tap indices: C0=0, CZk=1+3(k-1), CXk=2+3(k-1), CYk=3+3(k-1)
+  // (base/cpu_prop.cc:6-18)
+  g.taps.c0 = c->coeffs[0];
+  for (int k = 0; k < 4; k++) {
+    g.taps.cz[k] = c->coeffs[1 + 3 * k];
+    g.taps.cx[k] = c->coeffs[2 + 3 * k];
+    g.taps.cy[k] = c->coeffs[3 + 3 * k];
+  }
+  g.sponge.bx = c->d_bx;
+  g.sponge.by = c->d_by;
+  g.sponge.bz = c->d_bz;
+  return g;
+}
+
+born::RecMap recmap(const born_ctx *c) {
+  born::RecMap m;
+  m.plane_slot = c->d_plane_slot;
+  m.rmap = c->d_rmap;
+  m.rnext = c->d_rnext;
+  m.nrec = c->nrec;
+  m.ntb = c->nt_rec;
+  return m;
+}
+
+// Sponge factor per axis coordinate (cpu_prop.cc:141-165, 329-332).
+std::vector<float> axis_sponge(int n, int width, const std::vector<float> &bound) {
+  std::vector<float> b(n, 1.f);
+  for (int i = 4; i < n - 4; i++) {
+    int e = std::min(i - 4, n - 4 - i);
+    if (e >= 0 && e < width) b[i] = bound[e];
+  }
+  return b;
+}
+
+int upload_sponge(born_ctx *ctx) {
+  std::vector<float> bound(ctx->nbound);
+  const float bcA = (float)ctx->nbound;
+  for (int i = 0; i < ctx->nbound; i++) bound[i] = expf(-ctx->bcB * (bcA - i));
+  const int dims[3] = {ctx->nx, ctx->ny, ctx->nz};
+  float **dst[3] = {&ctx->d_bx, &ctx->d_by, &ctx->d_bz};
+  for (int a = 0; a < 3; a++) {
+    std::vector<float> b = axis_sponge(dims[a], ctx->nbound, bound);
+    CHK(dev_alloc(ctx, dst[a], b.size()));
+    HIPCHK(hipMemcpy(*dst[a], b.data(), b.size() * sizeof(float),
+                     hipMemcpyHostToDevice));
+  }
+  return BORN_OK;
+}
+
+bool interior(const born_ctx *c, long long loc, int *x, int *y, int *z) {
+  if (loc < 0 || loc >= c->n) return false;
+  const long long n12 = (long long)c->nx * c->ny;
+  *z = (int)(loc / n12);
+  *y = (int)((loc % n12) / c->nx);
+  *x = (int)(loc % c->nx);
+  return *x >= 4 && *x < c->nx - 4 && *y >= 4 && *y < c->ny - 4 && *z >= 4 &&
+         *z < c->nz - 4;
+}
+
+int need_space(born_ctx *ctx) {
+  if (!ctx) return BORN_ERR_ARG;
+  if (!ctx->have_space)
+    return fail(ctx, BORN_ERR_STATE, "born_create_space has not been called");
+  return BORN_OK;
+}
+
+int check_dims(born_ctx *ctx, int n1, int n2, int n3) {
+  if (n1 != ctx->nx || n2 != ctx->ny || n3 != ctx->nz)
+    return fail(ctx, BORN_ERR_ARG,
+                "grid " + std::to_string(n1) + "x" + std::to_string(n2) + "x" +
+                    std::to_string(n3) + " differs from the created space");
+  return BORN_OK;
+}
+
+int ensure_work(born_ctx *ctx) {
+  for (int i = 0; i < 4; i++)
+    if (!ctx->d_w[i]) CHK(dev_alloc(ctx, &ctx->d_w[i], (size_t)ctx->n));
+  return BORN_OK;
+}
+
+int zero(born_ctx *ctx, float *p, long long count) {
+  HIPCHK(born::launch_fill(p, count, 0.f, ctx->stream));
+  return BORN_OK;
+}
+
+// injection amplitudes of the source for nsteps steps (see src_amp_kernel)
+int source_amps(born_ctx *ctx, int nsteps, int jt_idx, int t_off, float scale) {
+  const size_t need = (size_t)std::max(nsteps, 1) * std::max(ctx->npts_s, 1);
+  CHK(ensure(ctx, &ctx->d_amp, &ctx->amp_cap, need));
+  HIPCHK(born::launch_src_amp(ctx->d_amp, nsteps, ctx->npts_s, ctx->d_src,
+                              ctx->nt_src, ctx->d_tab_s, ctx->nsinc_s, jt_idx,
+                              ctx->jts, t_off, scale, ctx->stream));
+  return BORN_OK;
+}
+
+int need_born_inputs(born_ctx *ctx, bool vel1, bool vel2, bool rec) {
+  CHK(need_space(ctx));
+  if (!ctx->d_tab_s || !ctx->d_tab_d)
+    return fail(ctx, BORN_ERR_STATE, "sinc tables have not been transferred");
+  if (!ctx->d_src)
+    return fail(ctx, BORN_ERR_STATE, "source function has not been transferred");
+  if (vel1 && !ctx->d_vel1)
+    return fail(ctx, BORN_ERR_STATE, "velocity 1 has not been transferred");
+  if (vel2 && !ctx->d_vel2)
+    return fail(ctx, BORN_ERR_STATE, "velocity 2 has not been transferred");
+  if (rec && !ctx->d_rmap)
+    return fail(ctx, BORN_ERR_STATE, "receivers have not been transferred");
+  return BORN_OK;
+}
+
+size_t free_bytes(born_ctx *ctx) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return fr;
+}
+
+// Bytes available for background history: the set budget, or the free
+// memory plus what the history buffers already hold, less a reserve.
+size_t history_budget(born_ctx *ctx) {
+  const size_t F = (size_t)ctx->n * sizeof(float);
+  if (ctx->budget) return ctx->budget;
+  const size_t held = (ctx->seg_cap + ctx->ckpt_cap) * F;
+  const size_t reserve = (size_t)2 << 30;
+  const size_t fr = free_bytes(ctx) + held;
+  return fr > reserve ? fr - reserve : 0;
+}
+
+// history buffers: L fields of scatter operand + 2*(S-1) checkpoint fields
+int history_alloc(born_ctx *ctx, size_t L, size_t nck) {
+  const size_t F = (size_t)ctx->n;
+  if (ctx->seg_cap < L) {
+    ctx->seg_cap = 0;
+    ctx->hist_version = -1;
+    CHK(dev_alloc(ctx, &ctx->d_seg, L * F));
+    ctx->seg_cap = L;
+  }
+  if (ctx->ckpt_cap < nck) {
+    ctx->ckpt_cap = 0;
+    CHK(dev_alloc(ctx, &ctx->d_ckpt, nck * F));
+    ctx->ckpt_cap = nck;
+  }
+  return BORN_OK;
+}
+
+int tr_buffers(born_ctx *ctx) {
+  const size_t need = (size_t)ctx->nrec * ctx->nt_rec;
+  if (ctx->tr_cap < need) {
+    ctx->tr_cap = 0;
+    CHK(dev_alloc(ctx, &ctx->d_tr, need));
+    CHK(dev_alloc(ctx, &ctx->d_tr2, need));
+    ctx->tr_cap = need;
+  }
+  return BORN_OK;
+}
+
+// host [nrec][ntb] -> device time-major [ntb][nrec] in ctx->d_tr
+int traces_in(born_ctx *ctx, const float *h) {
+  CHK(tr_buffers(ctx));
+  const size_t bytes = (size_t)ctx->nrec * ctx->nt_rec * sizeof(float);
+  HIPCHK(hipMemcpyAsync(ctx->d_tr2, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(born::launch_transpose(ctx->d_tr, ctx->d_tr2, ctx->nrec, ctx->nt_rec,
+                                ctx->stream));
+  return BORN_OK;
+}
+
+int traces_out(born_ctx *ctx, float *h) {
+  const size_t bytes = (size_t)ctx->nrec * ctx->nt_rec * sizeof(float);
+  HIPCHK(born::launch_transpose(ctx->d_tr2, ctx->d_tr, ctx->nt_rec, ctx->nrec,
+                                ctx->stream));
+  HIPCHK(hipMemcpyAsync(h, ctx->d_tr2, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BORN_OK;
+}
+
+int field_buf(born_ctx *ctx, float **p) {
+  if (!*p) CHK(dev_alloc(ctx, p, (size_t)ctx->n));
+  return BORN_OK;
+}
+
+int check_jt(born_ctx *ctx, int jt) {
+  // the forward indexes both sinc tables with the row it - (it/jt)*jt
+  // (cpu_prop.cc:31-41)
+  if (jt < 1 || jt > ctx->jts || jt > ctx->jtd)
+    return fail(ctx, BORN_ERR_UNSUPPORTED,
+                "jt=" + std::to_string(jt) + " exceeds the sinc tables (jts=" +
+                    std::to_string(ctx->jts) + ", jtd=" + std::to_string(ctx->jtd) +
+                    ")");
+  return BORN_OK;
+}
+
+}  // namespace
+
+// ======================================================================
+extern "C" {
+
+int born_create(born_ctx **out, int device) {
+  if (!out) return BORN_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return BORN_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) return BORN_ERR_ARG;
+  born_ctx *ctx = new born_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    delete ctx;
+    return BORN_ERR_HIP;
+  }
+  *out = ctx;
+  return BORN_OK;
+}
+
+int born_destroy(born_ctx *ctx) {
+  if (!ctx) return BORN_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  void *bufs[] = {ctx->d_bx,    ctx->d_by,    ctx->d_bz,   ctx->d_tab_s,
+                  ctx->d_tab_d, ctx->d_src,   ctx->d_vel1, ctx->d_vel2,
+                  ctx->d_plane_slot, ctx->d_rmap, ctx->d_rnext, ctx->d_w[0],
+                  ctx->d_w[1],  ctx->d_w[2],  ctx->d_w[3], ctx->d_img,
+                  ctx->d_acc,   ctx->d_p0,    ctx->d_p1,   ctx->d_tr,
+                  ctx->d_tr2,   ctx->d_amp,   ctx->d_seg,  ctx->d_ckpt};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return BORN_OK;
+}
+
+const char *born_last_error(const born_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int born_create_space(born_ctx *ctx, float d1, float d2, float d3, float bc_a,
+                      float bc_b, float bc_y, int nx, int ny, int nz) {
+  if (!ctx) return BORN_ERR_ARG;
+  (void)bc_a;
+  (void)bc_b;
+  (void)bc_y;
+  if (nx < 9 || ny < 9 || nz < 9)
+    return fail(ctx, BORN_ERR_ARG, "each axis needs at least 9 samples");
+  if (!(d1 > 0) || !(d2 > 0) || !(d3 > 0))
+    return fail(ctx, BORN_ERR_ARG, "sampling must be positive");
+  if ((long long)nx * ny * nz >= (1ll << 31))
+    return fail(ctx, BORN_ERR_UNSUPPORTED, "more than 2^31 cells");
+  CHK(set_device(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const bool resized = nx != ctx->nx || ny != ctx->ny || nz != ctx->nz;
+  ctx->nx = nx;
+  ctx->ny = ny;
+  ctx->nz = nz;
+  ctx->n = (long long)nx * ny * nz;
+  // cpu_prop.cc:296-326, double expressions rounded to float
+  auto C00 = [](double d) { return 8.0 / (5.0 * d * d); };
+  ctx->coeffs[0] = (float)(-1025.0 / 576.0 * (C00(d1) + C00(d2) + C00(d3)));
+  const double dd[3] = {d1, d2, d3};
+  const double f[4] = {1.0, -8.0, 63.0, -896.0};
+  for (int k = 0; k < 4; k++)
+    for (int a = 0; a < 3; a++)
+      ctx->coeffs[1 + 3 * k + a] = (float)(k == 0 ? C00(dd[a]) : C00(dd[a]) / f[k]);
+  ctx->nbound = 40;
+  ctx->bcB = .0005f;
+  if (resized) {
+    for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &ctx->d_w[i], 0));
+    CHK(dev_alloc(ctx, &ctx->d_img, 0));
+    CHK(dev_alloc(ctx, &ctx->d_acc, 0));
+    CHK(dev_alloc(ctx, &ctx->d_p0, 0));
+    CHK(dev_alloc(ctx, &ctx->d_p1, 0));
+    CHK(dev_alloc(ctx, &ctx->d_seg, 0));
+    CHK(dev_alloc(ctx, &ctx->d_ckpt, 0));
+    ctx->seg_cap = ctx->ckpt_cap = 0;
+    CHK(dev_alloc(ctx, &ctx->d_vel1, 0));
+    CHK(dev_alloc(ctx, &ctx->d_vel2, 0));
+    CHK(dev_alloc(ctx, &ctx->d_rmap, 0));
+    CHK(dev_alloc(ctx, &ctx->d_src, 0));
+  }
+  CHK(upload_sponge(ctx));
+  ctx->have_space = true;
+  ctx->bg_version++;
+  return BORN_OK;
+}
+
+int born_set_sponge(born_ctx *ctx, int width, float bcB) {
+  CHK(need_space(ctx));
+  if (width < 0 || width > 4096) return fail(ctx, BORN_ERR_ARG, "bad sponge width");
+  CHK(set_device(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->nbound = width;
+  ctx->bcB = bcB;
+  CHK(upload_sponge(ctx));
+  ctx->bg_version++;
+  return BORN_OK;
+}
+
+int born_set_ntblock(born_ctx *ctx, int nb) {
+  if (!ctx) return BORN_ERR_ARG;
+  (void)nb;
+  return BORN_OK;
+}
+
+static int sinc_table(born_ctx *ctx, int nsinc, int jt, const float *table,
+                      float **dst, int *ns_out, int *jt_out) {
+  if (!ctx) return BORN_ERR_ARG;
+  if (nsinc != 8) return fail(ctx, BORN_ERR_UNSUPPORTED, "nsinc must be 8");
+  if (jt < 1 || !table) return fail(ctx, BORN_ERR_ARG, "bad sinc table");
+  CHK(set_device(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  CHK(dev_alloc(ctx, dst, (size_t)nsinc * jt));
+  HIPCHK(hipMemcpy(*dst, table, (size_t)nsinc * jt * sizeof(float),
+                   hipMemcpyHostToDevice));
+  *ns_out = nsinc;
+  *jt_out = jt;
+  ctx->bg_version++;
+  return BORN_OK;
+}
+
+int born_transfer_sinc_table_s(born_ctx *ctx, int nsinc, int jts, const float *table) {
+  return sinc_table(ctx, nsinc, jts, table, ctx ? &ctx->d_tab_s : nullptr,
+                    ctx ? &ctx->nsinc_s : nullptr, ctx ? &ctx->jts : nullptr);
+}
+
+int born_transfer_sinc_table_d(born_ctx *ctx, int nsinc, int jtd, const float *table) {
+  return sinc_table(ctx, nsinc, jtd, table, ctx ? &ctx->d_tab_d : nullptr,
+                    ctx ? &ctx->nsinc_d : nullptr, ctx ? &ctx->jtd : nullptr);
+}
+
+int born_transfer_source_func(born_ctx *ctx, int npts, int nt_big, const int *locs,
+                              const float *vals) {
+  CHK(need_space(ctx));
+  if (npts < 0 || nt_big < 1 || (npts && (!locs || !vals)))
+    return fail(ctx, BORN_ERR_ARG, "bad source function");
+  if (npts > born::kMaxPts)
+    return fail(ctx, BORN_ERR_UNSUPPORTED,
+                "at most " + std::to_string(born::kMaxPts) + " source points");
+  born::SrcPts pts{};
+  pts.n = npts;
+  for (int i = 0; i < npts; i++)
+    if (!interior(ctx, locs[i], &pts.x[i], &pts.y[i], &pts.z[i]))
+      return fail(ctx, BORN_ERR_UNSUPPORTED,
+                  "source point " + std::to_string(i) + " outside the stencil interior");
+  CHK(set_device(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  CHK(dev_alloc(ctx, &ctx->d_src, (size_t)std::max(npts, 1) * nt_big));
+  if (npts)
+    HIPCHK(hipMemcpy(ctx->d_src, vals, (size_t)npts * nt_big * sizeof(float),
+                     hipMemcpyHostToDevice));
+  ctx->pts = pts;
+  ctx->npts_s = npts;
+  ctx->nt_src = nt_big;
+  ctx->bg_version++;
+  return BORN_OK;
+}
+
+static int vel_func(born_ctx *ctx, int nx, int ny, int nz, const float *v, float **dst) {
+  CHK(need_space(ctx));
+  CHK(check_dims(ctx, nx, ny, nz));
+  if (!v) return fail(ctx, BORN_ERR_ARG, "null velocity");
+  CHK(set_device(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  CHK(field_buf(ctx, dst));
+  HIPCHK(hipMemcpy(*dst, v, (size_t)ctx->n * sizeof(float), hipMemcpyHostToDevice));
+  ctx->bg_version++;
+  return BORN_OK;
+}
+
+int born_transfer_vel_func1(born_ctx *ctx, int nx, int ny, int nz, const float *v) {
+  return vel_func(ctx, nx, ny, nz, v, ctx ? &ctx->d_vel1 : nullptr);
+}
+
+int born_transfer_vel_func2(born_ctx *ctx, int nx, int ny, int nz, const float *v) {
+  return vel_func(ctx, nx, ny, nz, v, ctx ? &ctx->d_vel2 : nullptr);
+}
+
+int born_transfer_receiver_func(born_ctx *ctx, int rec_nx, int rec_ny, int nt_big,
+                                const int *locs, float *traces) {
+  CHK(need_space(ctx));
+  if (rec_nx < 1 || rec_ny < 1 || nt_big < 1 || !locs)
+    return fail(ctx, BORN_ERR_ARG, "bad receiver geometry");
+  const int nrec = rec_nx * rec_ny;
+  std::vector<int> px(nrec), py(nrec), pz(nrec);
+  std::vector<int> slot_of_z(ctx->nz, -1);
+  int nslots = 0;
+  for (int i = 0; i < nrec; i++) {
+    if (!interior(ctx, locs[i], &px[i], &py[i], &pz[i]))
+      return fail(ctx, BORN_ERR_UNSUPPORTED,
+                  "receiver " + std::to_string(i) + " outside the stencil interior");
+    if (slot_of_z[pz[i]] < 0) slot_of_z[pz[i]] = nslots++;
+  }
+  const long long n12 = (long long)ctx->nx * ctx->ny;
+  std::vector<int> rmap((size_t)nslots * n12, -1), rnext(nrec, -1), last((size_t)nslots * n12, -1);
+  for (int i = 0; i < nrec; i++) {
+    const long long c = slot_of_z[pz[i]] * n12 + (long long)py[i] * ctx->nx + px[i];
+    if (rmap[c] < 0)
+      rmap[c] = i;
+    else
+      rnext[last[c]] = i;
+    last[c] = i;
+  }
+  CHK(set_device(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  CHK(dev_alloc(ctx, &ctx->d_plane_slot, (size_t)ctx->nz));
+  CHK(dev_alloc(ctx, &ctx->d_rmap, rmap.size()));
+  CHK(dev_alloc(ctx, &ctx->d_rnext, rnext.size()));
+  HIPCHK(hipMemcpy(ctx->d_plane_slot, slot_of_z.data(), ctx->nz * sizeof(int),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->d_rmap, rmap.data(), rmap.size() * sizeof(int),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->d_rnext, rnext.data(), rnext.size() * sizeof(int),
+                   hipMemcpyHostToDevice));
+  ctx->rec_nx = rec_nx;
+  ctx->rec_ny = rec_ny;
+  ctx->nrec = nrec;
+  ctx->nt_rec = nt_big;
+  ctx->nslots = nslots;
+  ctx->h_rec = traces;
+  return BORN_OK;
+}
+
+// ------------------------------------------------------------- device loops
+
+int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *model,
+                     float *traces, int keep_background) {
+  CHK(need_born_inputs(ctx, false, true, true));
+  CHK(check_jt(ctx, jt));
+  if (nt < 0 || !model || !traces || (scatter != 0 && scatter != 1))
+    return fail(ctx, BORN_ERR_ARG, "bad forward arguments");
+  CHK(set_device(ctx));
+  CHK(ensure_work(ctx));
+  const long long n = ctx->n;
+  float *hist = nullptr;
+  if (keep_background && nt > 0) {
+    if ((size_t)nt * n * sizeof(float) <= history_budget(ctx)) {
+      CHK(history_alloc(ctx, (size_t)nt, 0));
+      hist = ctx->d_seg;
+    }
+  }
+  ctx->hist_version = -1;
+  ctx->jt_state = jt;
+  for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w[i], n));
+  CHK(source_amps(ctx, nt, jt, 0, 1 / (float)jt));
+  const Geometry geo = geometry(ctx);
+  float *s_old = ctx->d_w[0], *s_cur = ctx->d_w[1];
+  float *r_old = ctx->d_w[2], *r_cur = ctx->d_w[3];
+  born::FwdStep a{};
+  a.vel = ctx->d_vel2;
+  a.model = model;
+  a.src = ctx->pts;
+  a.rec = recmap(ctx);
+  a.tr = traces;
+  a.scatter = scatter;
+  for (int it = 0; it < nt; it++) {
+    const int id = it / jt, ii = it - id * jt;
+    a.s1 = s_cur;
+    a.s0 = s_old;
+    a.r1 = r_cur;
+    a.r0 = r_old;
+    a.hist = hist ? hist + (long long)it * n : nullptr;
+    a.amp = ctx->d_amp + (long long)it * ctx->npts_s;
+    a.tab = ctx->d_tab_d + (long long)ii * 8;
+    a.id = id;
+    a.extract = id + 7 < ctx->nt_rec;
+    HIPCHK(born::launch_fwd_step(geo, a, ctx->stream));
+    std::swap(s_old, s_cur);
+    std::swap(r_old, r_cur);
+  }
+  if (hist) {
+    ctx->hist_version = ctx->bg_version;
+    ctx->hist_scatter = scatter;
+    ctx->hist_jt = jt;
+    ctx->hist_nt = nt;
+  }
+  return BORN_OK;
+}
+
+int born_dev_source_prop(born_ctx *ctx, int jts, int nt, float *p0, float *p1) {
+  CHK(need_born_inputs(ctx, true, false, false));
+  if (nt < 0 || jts < 1 || !p0 || !p1) return fail(ctx, BORN_ERR_ARG, "bad source_prop arguments");
+  CHK(set_device(ctx));
+  // cpu_prop.cc:108-122: _jt = jts scales, the table's jts indexes
+  ctx->jt_state = jts;
+  CHK(source_amps(ctx, nt + 1, ctx->jts, 0, 1 / (float)jts));
+  const Geometry geo = geometry(ctx);
+  float *a0 = p0, *a1 = p1;
+  born::SrcStep s{};
+  s.vel = ctx->d_vel1;
+  s.src = ctx->pts;
+  for (int it = 0; it <= nt; it++) {
+    s.s1 = a1;
+    s.s0 = a0;
+    s.amp = ctx->d_amp + (long long)it * ctx->npts_s;
+    HIPCHK(born::launch_src_step(geo, s, ctx->stream));
+    std::swap(a0, a1);
+  }
+  if (nt % 2 == 1)  // cpu_prop.cc:123-128
+    HIPCHK(hipMemcpyAsync(p1, p0, (size_t)ctx->n * sizeof(float),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+  return BORN_OK;
+}
+
+int born_dev_rtm_adjoint(born_ctx *ctx, int nt, float *p0, float *p1, float *img,
+                         const float *traces) {
+  CHK(need_born_inputs(ctx, true, true, true));
+  if (nt < 0 || !p0 || !p1 || !img || !traces)
+    return fail(ctx, BORN_ERR_ARG, "bad rtm_adjoint arguments");
+  CHK(set_device(ctx));
+  CHK(ensure_work(ctx));
+  const long long n = ctx->n;
+  CHK(zero(ctx, ctx->d_w[0], n));
+  CHK(zero(ctx, ctx->d_w[1], n));
+  // cpu_prop.cc:71-95 with _dir = -1 and _jt from the last sourceProp
+  const float scale = -1 / (float)ctx->jt_state;
+  if (nt > 0) CHK(source_amps(ctx, nt, ctx->jts, 1, scale));
+  const Geometry geo = geometry(ctx);
+  float *s_old = p0, *s_cur = p1;
+  float *r_old = ctx->d_w[0], *r_cur = ctx->d_w[1];
+  born::RadjStep a{};
+  a.vel1 = ctx->d_vel1;
+  a.vel2 = ctx->d_vel2;
+  a.img = img;
+  a.src = ctx->pts;
+  a.rec = recmap(ctx);
+  a.tr = traces;
+  a.sc = (float)-1 / (float)ctx->jt_state;
+  for (int it = nt - 1; it >= 0; it--) {
+    const int id = it / ctx->jtd, ii = it - id * ctx->jtd;
+    a.s1 = s_cur;
+    a.s0 = s_old;
+    a.r1 = r_cur;
+    a.r0 = r_old;
+    a.s_prop = it < nt - 1;
+    a.r_prop = it > 0;
+    a.pre = it + 1 < nt - 1;
+    a.amp_pre = a.pre ? ctx->d_amp + (long long)(it + 1) * ctx->npts_s : ctx->d_amp;
+    a.tab = ctx->d_tab_d + (long long)ii * 8;
+    a.id = id;
+    a.inject = id + 7 < ctx->nt_rec;
+    HIPCHK(born::launch_radj_step(geo, a, ctx->stream));
+    std::swap(s_old, s_cur);
+    std::swap(r_old, r_cur);
+  }
+  return BORN_OK;
+}
+
+int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
+                           const float *traces, float *img) {
+  CHK(need_born_inputs(ctx, false, true, true));
+  CHK(check_jt(ctx, jt));
+  if (nt < 0 || !traces || !img || (scatter != 0 && scatter != 1))
+    return fail(ctx, BORN_ERR_ARG, "bad adjoint arguments");
+  CHK(set_device(ctx));
+  CHK(ensure_work(ctx));
+  CHK(field_buf(ctx, &ctx->d_acc));
+  const long long n = ctx->n;
+  if (nt == 0) return BORN_OK;
+  const Geometry geo = geometry(ctx);
+  const bool cached = ctx->hist_version == ctx->bg_version &&
+                      ctx->hist_scatter == scatter && ctx->hist_jt == jt &&
+                      ctx->hist_nt == nt && ctx->seg_cap >= (size_t)nt;
+  int S = 1, L = nt;
+  if (!cached) {
+    ctx->hist_version = -1;  // the segment buffer is about to be reused
+    const size_t F = (size_t)n * sizeof(float);
+    const size_t avail = history_budget(ctx);
+    S = 0;
+    for (int s = 1; s <= nt; s++) {
+      const size_t l = (size_t)((nt + s - 1) / s);
+      if ((l + 2 * (size_t)(s - 1)) * F <= avail) {
+        S = s;
+        break;
+      }
+      if (l <= 2) break;
+    }
+    if (S == 0)
+      return fail(ctx, BORN_ERR_NOMEM, "no checkpoint plan fits the HBM budget");
+    L = (nt + S - 1) / S;
+    S = (nt + L - 1) / L;
+    CHK(history_alloc(ctx, (size_t)L, 2 * (size_t)(S - 1)));
+  }
+  CHK(source_amps(ctx, nt, jt, 0, 1 / (float)jt));
+  float *s_old = ctx->d_w[2], *s_cur = ctx->d_w[3];
+  born::SrecStep sr{};
+  sr.vel = ctx->d_vel2;
+  sr.src = ctx->pts;
+  sr.scatter = scatter;
+  const size_t Fb = (size_t)n * sizeof(float);
+  int recomputed = 0;
+  if (!cached) {
+    // pass 1: background sweep; checkpoints at segment starts, the last
+    // segment's scatter operands stored directly
+    CHK(zero(ctx, s_old, n));
+    CHK(zero(ctx, s_cur, n));
+    const int klast = (S - 1) * L;
+    for (int k = 0; k < nt; k++) {
+      if (k > 0 && k % L == 0 && k / L <= S - 1) {
+        float *ck = ctx->d_ckpt + (long long)2 * (k / L - 1) * n;
+        HIPCHK(hipMemcpyAsync(ck, s_cur, Fb, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ck + n, s_old, Fb, hipMemcpyDeviceToDevice, ctx->stream));
+      }
+      sr.s1 = s_cur;
+      sr.s0 = s_old;
+      sr.sig = k >= klast ? ctx->d_seg + (long long)(k - klast) * n : nullptr;
+      sr.amp = ctx->d_amp + (long long)k * ctx->npts_s;
+      HIPCHK(born::launch_srec_step(geo, sr, ctx->stream));
+      std::swap(s_old, s_cur);
+    }
+  }
+  // pass 2: adjoint state, segment by segment in reverse
+  float *u_old = ctx->d_w[0], *u_cur = ctx->d_w[1];
+  CHK(zero(ctx, u_old, n));
+  CHK(zero(ctx, u_cur, n));
+  CHK(zero(ctx, ctx->d_acc, n));
+  born::LamStep lm{};
+  lm.vel = ctx->d_vel2;
+  lm.acc = ctx->d_acc;
+  lm.rec = recmap(ctx);
+  lm.tr = traces;
+  for (int sg = S - 1; sg >= 0; sg--) {
+    const int k0 = sg * L, k1 = std::min(nt, k0 + L);
+    if (sg < S - 1) {
+      if (sg == 0) {
+        CHK(zero(ctx, s_old, n));
+        CHK(zero(ctx, s_cur, n));
+      } else {
+        const float *ck = ctx->d_ckpt + (long long)2 * (sg - 1) * n;
+        HIPCHK(hipMemcpyAsync(s_cur, ck, Fb, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(s_old, ck + n, Fb, hipMemcpyDeviceToDevice, ctx->stream));
+      }
+      for (int k = k0; k < k1; k++) {
+        sr.s1 = s_cur;
+        sr.s0 = s_old;
+        sr.sig = ctx->d_seg + (long long)(k - k0) * n;
+        sr.amp = ctx->d_amp + (long long)k * ctx->npts_s;
+        HIPCHK(born::launch_srec_step(geo, sr, ctx->stream));
+        std::swap(s_old, s_cur);
+        recomputed++;
+      }
+    }
+    const float *seg = ctx->d_seg;
+    for (int k = k1 - 1; k >= k0; k--) {
+      const int id = k / jt, ii = k - id * jt;
+      lm.u1 = u_cur;
+      lm.u0 = u_old;
+      lm.sig = seg + (long long)(k - k0) * n;
+      lm.tab = ctx->d_tab_d + (long long)ii * 8;
+      lm.id = id;
+      lm.inject = id + 7 < ctx->nt_rec;
+      HIPCHK(born::launch_lam_step(geo, lm, ctx->stream));
+      std::swap(u_old, u_cur);
+    }
+  }
+  HIPCHK(born::launch_adj_finish(geo, img, ctx->d_acc, ctx->d_vel2,
+                                 scatter == 0 ? 1e-6f : 1.f, ctx->stream));
+  ctx->last_segments = S;
+  ctx->last_recomputed = recomputed;
+  return BORN_OK;
+}
+
+// ----------------------------------------------------- host-pointer entries
+
+int born_rtm_forward(born_ctx *ctx, int n1, int n2, int n3, int jt, const float *img,
+                     float *rec, int npts, int nt, int nt_big, int rec_nx, int rec_ny) {
+  (void)rec;
+  (void)npts;
+  CHK(need_space(ctx));
+  CHK(check_dims(ctx, n1, n2, n3));
+  if (!img) return fail(ctx, BORN_ERR_ARG, "null image");
+  if (!ctx->d_rmap || !ctx->h_rec)
+    return fail(ctx, BORN_ERR_STATE, "receivers have not been transferred");
+  if (nt_big != ctx->nt_rec || rec_nx * rec_ny != ctx->nrec)
+    return fail(ctx, BORN_ERR_ARG, "trace geometry differs from the transferred one");
+  CHK(set_device(ctx));
+  CHK(field_buf(ctx, &ctx->d_img));
+  HIPCHK(hipMemcpyAsync(ctx->d_img, img, (size_t)ctx->n * sizeof(float),
+                        hipMemcpyHostToDevice, ctx->stream));
+  CHK(traces_in(ctx, ctx->h_rec));
+  CHK(born_dev_forward(ctx, jt, nt, 0, ctx->d_img, ctx->d_tr, 0));
+  CHK(traces_out(ctx, ctx->h_rec));
+  return BORN_OK;
+}
+
+int born_forward_scatter(born_ctx *ctx, int n1, int n2, int n3, int jt,
+                         const float *img, float *rec, int nt, int scatter) {
+  CHK(need_space(ctx));
+  CHK(check_dims(ctx, n1, n2, n3));
+  if (!img || !rec) return fail(ctx, BORN_ERR_ARG, "null argument");
+  if (!ctx->d_rmap) return fail(ctx, BORN_ERR_STATE, "receivers have not been transferred");
+  CHK(set_device(ctx));
+  CHK(field_buf(ctx, &ctx->d_img));
+  HIPCHK(hipMemcpyAsync(ctx->d_img, img, (size_t)ctx->n * sizeof(float),
+                        hipMemcpyHostToDevice, ctx->stream));
+  CHK(traces_in(ctx, rec));
+  CHK(born_dev_forward(ctx, jt, nt, scatter, ctx->d_img, ctx->d_tr, 0));
+  CHK(traces_out(ctx, rec));
+  return BORN_OK;
+}
+
+int born_source_prop(born_ctx *ctx, int nx, int ny, int nz, int damp, int get_last,
+                     float *p0, float *p1, int jts, int npts, int nt) {
+  (void)damp;
+  (void)get_last;
+  (void)npts;
+  CHK(need_space(ctx));
+  CHK(check_dims(ctx, nx, ny, nz));
+  if (!p0 || !p1) return fail(ctx, BORN_ERR_ARG, "null state");
+  CHK(set_device(ctx));
+  CHK(field_buf(ctx, &ctx->d_p0));
+  CHK(field_buf(ctx, &ctx->d_p1));
+  const size_t B = (size_t)ctx->n * sizeof(float);
+  HIPCHK(hipMemcpyAsync(ctx->d_p0, p0, B, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->d_p1, p1, B, hipMemcpyHostToDevice, ctx->stream));
+  CHK(born_dev_source_prop(ctx, jts, nt, ctx->d_p0, ctx->d_p1));
+  HIPCHK(hipMemcpyAsync(p0, ctx->d_p0, B, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(p1, ctx->d_p1, B, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BORN_OK;
+}
+
+int born_rtm_adjoint(born_ctx *ctx, int n1, int n2, int n3, int jtd, float *p0,
+                     float *p1, float *img, int npts_s, int nt) {
+  (void)jtd;
+  (void)npts_s;
+  CHK(need_space(ctx));
+  CHK(check_dims(ctx, n1, n2, n3));
+  if (!p0 || !p1 || !img) return fail(ctx, BORN_ERR_ARG, "null argument");
+  if (!ctx->d_rmap || !ctx->h_rec)
+    return fail(ctx, BORN_ERR_STATE, "receivers have not been transferred");
+  CHK(set_device(ctx));
+  CHK(field_buf(ctx, &ctx->d_p0));
+  CHK(field_buf(ctx, &ctx->d_p1));
+  CHK(field_buf(ctx, &ctx->d_img));
+  const size_t B = (size_t)ctx->n * sizeof(float);
+  HIPCHK(hipMemcpyAsync(ctx->d_p0, p0, B, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->d_p1, p1, B, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->d_img, img, B, hipMemcpyHostToDevice, ctx->stream));
+  CHK(traces_in(ctx, ctx->h_rec));
+  CHK(born_dev_rtm_adjoint(ctx, nt, ctx->d_p0, ctx->d_p1, ctx->d_img, ctx->d_tr));
+  HIPCHK(hipMemcpyAsync(img, ctx->d_img, B, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(p0, ctx->d_p0, B, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(p1, ctx->d_p1, B, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BORN_OK;
+}
+
+int born_adjoint_exact(born_ctx *ctx, int n1, int n2, int n3, int jt, const float *rec,
+                       float *img, int nt, int scatter) {
+  CHK(need_space(ctx));
+  CHK(check_dims(ctx, n1, n2, n3));
+  if (!rec || !img) return fail(ctx, BORN_ERR_ARG, "null argument");
+  if (!ctx->d_rmap) return fail(ctx, BORN_ERR_STATE, "receivers have not been transferred");
+  CHK(set_device(ctx));
+  CHK(field_buf(ctx, &ctx->d_img));
+  const size_t B = (size_t)ctx->n * sizeof(float);
+  HIPCHK(hipMemcpyAsync(ctx->d_img, img, B, hipMemcpyHostToDevice, ctx->stream));
+  CHK(traces_in(ctx, rec));
+  CHK(born_dev_adjoint_exact(ctx, jt, nt, scatter, ctx->d_tr, ctx->d_img));
+  HIPCHK(hipMemcpyAsync(img, ctx->d_img, B, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BORN_OK;
+}
+
+int born_set_checkpoint_budget(born_ctx *ctx, size_t bytes) {
+  if (!ctx) return BORN_ERR_ARG;
+  ctx->budget = bytes;
+  return BORN_OK;
+}
+
+int born_last_checkpoint_plan(const born_ctx *ctx, int *segments, int *recomputed) {
+  if (!ctx) return BORN_ERR_ARG;
+  if (segments) *segments = ctx->last_segments;
+  if (recomputed) *recomputed = ctx->last_recomputed;
+  return BORN_OK;
+}
+
+int born_set_zchunk(born_ctx *ctx, int zchunk) {
+  if (!ctx || zchunk < 0) return BORN_ERR_ARG;
+  ctx->zchunk = zchunk;
+  return BORN_OK;
+}
+
+void *born_get_stream(const born_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int born_synchronize(born_ctx *ctx) {
+  if (!ctx) return BORN_ERR_ARG;
+  CHK(set_device(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BORN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,423 @@
+// Fused time-step kernels of the Born propagator (gfx950).
+//
+// Each op below is one time step of one of the reference's loops
+// (base/cpu_prop.cc), with every per-cell pass of that step fused into the
+// single z-marching sweep of march_kernel:
+//   FwdOp     rtmForward step (cpu_prop.cc:31-53): prop s, prop r, damp both,
+//             inject source, Born scatter, extract receivers.
+//   SrcOp     sourceProp step (cpu_prop.cc:112-122): undamped, inject source.
+//   RadjOp    rtmAdjoint step (cpu_prop.cc:71-95): reverse s (vel1), r (vel2),
+//             damp r, inject receivers, imaging condition.
+//   SrecOp    background step of the exact adjoint (the s half of FwdOp) that
+//             also stores the scatter operand of the step.
+//   LamOp     adjoint-state step of the exact adjoint (see born_abi.hip).
+//
+// Storage convention: a field buffer holds each level exactly as the
+// reference holds it right after that level was produced (damped, injected).
+// The reference additionally damps the current level in place after the
+// update (cpu_prop.cc:141-165 on p1); that value is only ever read back as the
+// "old" level of the next step, so it is recomputed there as fl(b * old)
+// instead of being written: same arithmetic, one HBM write less per field.
+#include "born_device.h"
+#include "born_launch.h"
+
+namespace born {
+
+struct RecvCtx {
+  RecMap rm;
+  int slot;  // receiver slot of the current plane (-1: none)
+  __device__ void plane(int z) { slot = rm.plane_slot[z]; }
+  __device__ int first(int x, int y, int nx, int ny) const {
+    return rm.rmap[((long long)slot * ny + y) * nx + x];
+  }
+};
+
+// ---------------------------------------------------------------- forward
+template <int SCATTER>
+struct FwdOp {
+  static constexpr int NF = 2;
+  static constexpr bool DAMP = true;
+  const float *s1, *r1;
+  float *s0, *r0;
+  const float *vel, *model;
+  float *hist;        // optional scatter-operand history (nullptr: none)
+  PtList pts;
+  const float *amp;   // source amplitudes of this step [npts]
+  RecvCtx rc;
+  float *tr;          // traces, time-major [ntb][nrec]
+  const float *tab;   // receiver sinc row of this step [8]
+  int id;             // trace time index of this step
+  int extract;        // id + 7 < ntb
+  int nx, ny;
+  SrcInBlock sb;
+  uint32_t pm;
+
+  __device__ const float *cur(int f) const { return f == 0 ? s1 : r1; }
+  __device__ void block_init(int x0, int x1, int y0, int y1, int z0, int z1) {
+    sb.init(pts, x0, x1, y0, y1, z0, z1);
+  }
+  __device__ void plane(int z) {
+    pm = sb.on_plane(pts, z);
+    rc.plane(z);
+  }
+  __device__ void cell(const CellArgs &c, const float *lap, const float *ctr) {
+    const float v = vel[c.idx];
+    const float b = c.b;
+    float sn = b * (v * lap[0] + ctr[0] + ctr[0] - b * s0[c.idx]);
+    if (pm) sn = SrcInBlock::add(pts, pm, c.x, c.y, sn, amp);
+    float rn = b * (v * lap[1] + ctr[1] + ctr[1] - b * r0[c.idx]);
+    float sig;
+    if (SCATTER == 0) {
+      // the reference's double literal: evaluated in double, rounded once
+      // (cpu_prop.cc:220)
+      rn = (float)((double)rn + .000001 * (double)sn * (double)model[c.idx]);
+      sig = sn;
+    } else {
+      sig = b * lap[0];
+      rn = rn + model[c.idx] * sig;
+    }
+    s0[c.idx] = sn;
+    r0[c.idx] = rn;
+    if (hist) hist[c.idx] = sig;
+    if (rc.slot >= 0 && extract) {
+      for (int i = rc.first(c.x, c.y, nx, ny); i >= 0; i = rc.rm.rnext[i]) {
+        float *p = tr + (long long)id * rc.rm.nrec + i;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          p[(long long)k * rc.rm.nrec] =
+              __fadd_rn(p[(long long)k * rc.rm.nrec], __fmul_rn(rn, tab[k]));
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------- sourceProp
+struct SrcOp {
+  static constexpr int NF = 1;
+  static constexpr bool DAMP = false;
+  const float *s1;
+  float *s0;
+  const float *vel;
+  PtList pts;
+  const float *amp;
+  SrcInBlock sb;
+  uint32_t pm;
+  __device__ const float *cur(int) const { return s1; }
+  __device__ void block_init(int x0, int x1, int y0, int y1, int z0, int z1) {
+    sb.init(pts, x0, x1, y0, y1, z0, z1);
+  }
+  __device__ void plane(int z) { pm = sb.on_plane(pts, z); }
+  __device__ void cell(const CellArgs &c, const float *lap, const float *ctr) {
+    float sn = vel[c.idx] * lap[0] + ctr[0] + ctr[0] - s0[c.idx];
+    if (pm) sn = SrcInBlock::add(pts, pm, c.x, c.y, sn, amp);
+    s0[c.idx] = sn;
+  }
+};
+
+// ------------------------------------------------------ reference adjoint
+struct RadjOp {
+  static constexpr int NF = 2;
+  static constexpr bool DAMP = true;
+  const float *s1, *r1;
+  float *s0, *r0;
+  const float *vel1, *vel2;
+  float *img;
+  int s_prop, r_prop;
+  // source amplitude injected into the current level by the previous
+  // iteration (cpu_prop.cc:89-92), applied here to the old level it became
+  PtList pts;
+  const float *amp_pre;
+  int pre;
+  RecvCtx rc;
+  const float *tr;    // traces, time-major [ntb][nrec]
+  const float *tab;   // receiver sinc row [8]
+  float sc;           // (float)dir / (float)jt
+  int id, inject;
+  int nx, ny;
+  SrcInBlock sb;
+  uint32_t pm;
+
+  __device__ const float *cur(int f) const { return f == 0 ? s1 : r1; }
+  __device__ void block_init(int x0, int x1, int y0, int y1, int z0, int z1) {
+    sb.init(pts, x0, x1, y0, y1, z0, z1);
+  }
+  __device__ void plane(int z) {
+    pm = pre ? sb.on_plane(pts, z) : 0u;
+    rc.plane(z);
+  }
+  __device__ void cell(const CellArgs &c, const float *lap, const float *ctr) {
+    float sn;
+    if (s_prop) {
+      float so = s0[c.idx];
+      if (pm) so = SrcInBlock::add(pts, pm, c.x, c.y, so, amp_pre);
+      sn = vel1[c.idx] * lap[0] + ctr[0] + ctr[0] - so;
+      s0[c.idx] = sn;
+    } else {
+      sn = s0[c.idx];
+    }
+    const float b = c.b;
+    const float ro = r0[c.idx];
+    float rn = r_prop ? b * (vel2[c.idx] * lap[1] + ctr[1] + ctr[1] - b * ro)
+                      : b * (b * ro);
+    if (rc.slot >= 0 && inject) {
+      for (int i = rc.first(c.x, c.y, nx, ny); i >= 0; i = rc.rm.rnext[i])
+        rn = __fadd_rn(rn, __fmul_rn(sc, sinc8(tab, tr + (long long)id * rc.rm.nrec + i,
+                                               rc.rm.nrec)));
+    }
+    r0[c.idx] = rn;
+    img[c.idx] += sn * rn;
+  }
+};
+
+// ------------------------------------- exact adjoint: background recompute
+template <int SCATTER>
+struct SrecOp {
+  static constexpr int NF = 1;
+  static constexpr bool DAMP = true;
+  const float *s1;
+  float *s0;
+  const float *vel;
+  float *sig;  // scatter operand of this step
+  PtList pts;
+  const float *amp;
+  SrcInBlock sb;
+  uint32_t pm;
+  __device__ const float *cur(int) const { return s1; }
+  __device__ void block_init(int x0, int x1, int y0, int y1, int z0, int z1) {
+    sb.init(pts, x0, x1, y0, y1, z0, z1);
+  }
+  __device__ void plane(int z) { pm = sb.on_plane(pts, z); }
+  __device__ void cell(const CellArgs &c, const float *lap, const float *ctr) {
+    const float b = c.b;
+    float sn = b * (vel[c.idx] * lap[0] + ctr[0] + ctr[0] - b * s0[c.idx]);
+    if (pm) sn = SrcInBlock::add(pts, pm, c.x, c.y, sn, amp);
+    s0[c.idx] = sn;
+    if (sig) sig[c.idx] = SCATTER == 0 ? sn : b * lap[0];
+  }
+};
+
+// ---------------------------------------- exact adjoint: adjoint-state step
+// nu = V B lambda obeys the forward recursion with V B-weighted receiver
+// injection; the image accumulates sigma * nu and is divided by V B at the end.
+struct LamOp {
+  static constexpr int NF = 1;
+  static constexpr bool DAMP = true;
+  const float *u1;
+  float *u0;
+  const float *vel;
+  const float *sig;
+  float *acc;
+  RecvCtx rc;
+  const float *tr;
+  const float *tab;
+  int id, inject;
+  int nx, ny;
+  __device__ const float *cur(int) const { return u1; }
+  __device__ void block_init(int, int, int, int, int, int) {}
+  __device__ void plane(int z) { rc.plane(z); }
+  __device__ void cell(const CellArgs &c, const float *lap, const float *ctr) {
+    const float b = c.b;
+    const float v = vel[c.idx];
+    float un = b * (v * lap[0] + ctr[0] + ctr[0] - b * u0[c.idx]);
+    if (rc.slot >= 0 && inject) {
+      float g = 0.f;
+      for (int i = rc.first(c.x, c.y, nx, ny); i >= 0; i = rc.rm.rnext[i])
+        g += sinc8(tab, tr + (long long)id * rc.rm.nrec + i, rc.rm.nrec);
+      un += v * b * g;
+    }
+    u0[c.idx] = un;
+    acc[c.idx] += sig[c.idx] * un;
+  }
+};
+
+// ------------------------------------------------------------ small kernels
+
+__global__ void fill_kernel(float *p, long long n, float v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// amp[s*npts + i] = scale * sinc8(tab[ii], src[i][id..id+7]) with
+// tt = s + t_off, id = tt / jt, ii = tt - id*jt; zero when id + 7 >= ntsrc
+// (the guard of cpu_prop.cc:167).
+__global__ void src_amp_kernel(float *amp, int nsteps, int npts,
+                               const float *src, int ntsrc, const float *tab,
+                               int nsinc, int jt, int ntab, int t_off,
+                               float scale) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsteps * npts) return;
+  const int s = t / npts, i = t % npts;
+  const int tt = s + t_off;
+  const int id = tt / jt;
+  const int ii = tt - id * jt;
+  float a = 0.f;
+  if (id + 7 < ntsrc && ii < ntab)
+    a = __fmul_rn(scale, sinc8(tab + (long long)ii * nsinc,
+                               src + (long long)i * ntsrc + id, 1));
+  amp[t] = a;
+}
+
+// out[i] += scale * acc[i] / (vel[i] * b[i]) on the interior.
+__global__ void adj_finish_kernel(float *out, const float *acc,
+                                  const float *vel, const Sponge sp, int nx,
+                                  int ny, int nz, float scale) {
+  const long long n12 = (long long)nx * ny;
+  const long long n = n12 * nz;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / n12);
+    if (x < kH || x >= nx - kH || y < kH || y >= ny - kH || z < kH ||
+        z >= nz - kH)
+      continue;
+    const float b = fminf(fminf(sp.bx[x], sp.by[y]), sp.bz[z]);
+    out[i] += scale * acc[i] / (vel[i] * b);
+  }
+}
+
+// [n0][n1] -> [n1][n0]
+__global__ void transpose_kernel(float *dst, const float *src, int n0, int n1) {
+  __shared__ float t[32][33];
+  const int b0 = blockIdx.y * 32, b1 = blockIdx.x * 32;
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    const int i0 = b0 + k, i1 = b1 + threadIdx.x;
+    if (i0 < n0 && i1 < n1) t[k][threadIdx.x] = src[(long long)i0 * n1 + i1];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    const int i1 = b1 + k, i0 = b0 + threadIdx.x;
+    if (i0 < n0 && i1 < n1) dst[(long long)i1 * n0 + i0] = t[threadIdx.x][k];
+  }
+}
+
+// ------------------------------------------------------------- launchers
+
+static Grid make_grid(const Geometry &geo) {
+  Grid g;
+  g.nx = geo.nx;
+  g.ny = geo.ny;
+  g.nz = geo.nz;
+  g.n12 = (long long)geo.nx * geo.ny;
+  g.ntx = (geo.nx - 2 * kH + kTX - 1) / kTX;
+  g.nty = (geo.ny - 2 * kH + kTY - 1) / kTY;
+  g.zc = geo.zchunk > 0 ? geo.zchunk : 32;
+  g.ntz = (geo.nz - 2 * kH + g.zc - 1) / g.zc;
+  return g;
+}
+
+template <class Op>
+static hipError_t launch(const Geometry &geo, const Op &op, hipStream_t st) {
+  const Grid g = make_grid(geo);
+  const int nb = g.ntx * g.nty * g.ntz;
+  hipLaunchKernelGGL((march_kernel<Op>), dim3(nb), dim3(kThreads), 0, st, g,
+                     geo.taps, geo.sponge, op);
+  return hipGetLastError();
+}
+
+static PtList to_pts(const SrcPts &s) {
+  PtList p;
+  p.n = s.n;
+  for (int i = 0; i < kMaxPts; i++) {
+    p.x[i] = i < s.n ? s.x[i] : -1;
+    p.y[i] = i < s.n ? s.y[i] : -1;
+    p.z[i] = i < s.n ? s.z[i] : -1;
+  }
+  return p;
+}
+
+hipError_t launch_fwd_step(const Geometry &geo, const FwdStep &a, hipStream_t st) {
+  if (a.scatter == 0) {
+    FwdOp<0> op{};
+    op.s1 = a.s1; op.r1 = a.r1; op.s0 = a.s0; op.r0 = a.r0;
+    op.vel = a.vel; op.model = a.model; op.hist = a.hist;
+    op.pts = to_pts(a.src); op.amp = a.amp;
+    op.rc.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
+    op.extract = a.extract; op.nx = geo.nx; op.ny = geo.ny;
+    return launch(geo, op, st);
+  }
+  FwdOp<1> op{};
+  op.s1 = a.s1; op.r1 = a.r1; op.s0 = a.s0; op.r0 = a.r0;
+  op.vel = a.vel; op.model = a.model; op.hist = a.hist;
+  op.pts = to_pts(a.src); op.amp = a.amp;
+  op.rc.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
+  op.extract = a.extract; op.nx = geo.nx; op.ny = geo.ny;
+  return launch(geo, op, st);
+}
+
+hipError_t launch_src_step(const Geometry &geo, const SrcStep &a, hipStream_t st) {
+  SrcOp op{};
+  op.s1 = a.s1; op.s0 = a.s0; op.vel = a.vel;
+  op.pts = to_pts(a.src); op.amp = a.amp;
+  return launch(geo, op, st);
+}
+
+hipError_t launch_radj_step(const Geometry &geo, const RadjStep &a, hipStream_t st) {
+  RadjOp op{};
+  op.s1 = a.s1; op.r1 = a.r1; op.s0 = a.s0; op.r0 = a.r0;
+  op.vel1 = a.vel1; op.vel2 = a.vel2; op.img = a.img;
+  op.s_prop = a.s_prop; op.r_prop = a.r_prop;
+  op.pts = to_pts(a.src); op.amp_pre = a.amp_pre; op.pre = a.pre;
+  op.rc.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.sc = a.sc;
+  op.id = a.id; op.inject = a.inject; op.nx = geo.nx; op.ny = geo.ny;
+  return launch(geo, op, st);
+}
+
+hipError_t launch_srec_step(const Geometry &geo, const SrecStep &a, hipStream_t st) {
+  if (a.scatter == 0) {
+    SrecOp<0> op{};
+    op.s1 = a.s1; op.s0 = a.s0; op.vel = a.vel; op.sig = a.sig;
+    op.pts = to_pts(a.src); op.amp = a.amp;
+    return launch(geo, op, st);
+  }
+  SrecOp<1> op{};
+  op.s1 = a.s1; op.s0 = a.s0; op.vel = a.vel; op.sig = a.sig;
+  op.pts = to_pts(a.src); op.amp = a.amp;
+  return launch(geo, op, st);
+}
+
+hipError_t launch_lam_step(const Geometry &geo, const LamStep &a, hipStream_t st) {
+  LamOp op{};
+  op.u1 = a.u1; op.u0 = a.u0; op.vel = a.vel; op.sig = a.sig; op.acc = a.acc;
+  op.rc.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
+  op.inject = a.inject; op.nx = geo.nx; op.ny = geo.ny;
+  return launch(geo, op, st);
+}
+
+hipError_t launch_fill(float *p, long long n, float v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)nb), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_src_amp(float *amp, int nsteps, int npts, const float *src,
+                          int ntsrc, const float *tab, int nsinc, int jt,
+                          int ntab, int t_off, float scale, hipStream_t st) {
+  const int n = nsteps * npts;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(src_amp_kernel, dim3((n + 255) / 256), dim3(256), 0, st,
+                     amp, nsteps, npts, src, ntsrc, tab, nsinc, jt, ntab,
+                     t_off, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
+                             const float *vel, float scale, hipStream_t st) {
+  const long long n = (long long)geo.nx * geo.ny * geo.nz;
+  long long nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(adj_finish_kernel, dim3((unsigned)nb), dim3(256), 0, st,
+                     out, acc, vel, geo.sponge, geo.nx, geo.ny, geo.nz, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(float *dst, const float *src, int n0, int n1,
+                            hipStream_t st) {
+  if (n0 <= 0 || n1 <= 0) return hipSuccess;
+  hipLaunchKernelGGL(transpose_kernel, dim3((n1 + 31) / 32, (n0 + 31) / 32),
+                     dim3(32, 8), 0, st, dst, src, n0, n1);
+  return hipGetLastError();
+}
+
+}  // namespace born

@@ -1,0 +1,93 @@
+// Host-side launch interface of the fused step kernels (born_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "born_device.h"
+
+namespace born {
+
+struct Geometry {
+  int nx, ny, nz;
+  int zchunk;  // z planes per block (0: default)
+  Taps taps;
+  Sponge sponge;
+};
+
+struct SrcPts {
+  int n;
+  int x[kMaxPts], y[kMaxPts], z[kMaxPts];
+};
+
+struct FwdStep {
+  const float *s1, *r1;
+  float *s0, *r0;
+  const float *vel, *model;
+  float *hist;
+  SrcPts src;
+  const float *amp;
+  RecMap rec;
+  float *tr;
+  const float *tab;
+  int id, extract, scatter;
+};
+
+struct SrcStep {
+  const float *s1;
+  float *s0;
+  const float *vel;
+  SrcPts src;
+  const float *amp;
+};
+
+struct RadjStep {
+  const float *s1, *r1;
+  float *s0, *r0;
+  const float *vel1, *vel2;
+  float *img;
+  int s_prop, r_prop;
+  SrcPts src;
+  const float *amp_pre;
+  int pre;
+  RecMap rec;
+  const float *tr;
+  const float *tab;
+  float sc;
+  int id, inject;
+};
+
+struct SrecStep {
+  const float *s1;
+  float *s0;
+  const float *vel;
+  float *sig;
+  SrcPts src;
+  const float *amp;
+  int scatter;
+};
+
+struct LamStep {
+  const float *u1;
+  float *u0;
+  const float *vel, *sig;
+  float *acc;
+  RecMap rec;
+  const float *tr;
+  const float *tab;
+  int id, inject;
+};
+
+hipError_t launch_fwd_step(const Geometry &geo, const FwdStep &a, hipStream_t st);
+hipError_t launch_src_step(const Geometry &geo, const SrcStep &a, hipStream_t st);
+hipError_t launch_radj_step(const Geometry &geo, const RadjStep &a, hipStream_t st);
+hipError_t launch_srec_step(const Geometry &geo, const SrecStep &a, hipStream_t st);
+hipError_t launch_lam_step(const Geometry &geo, const LamStep &a, hipStream_t st);
+hipError_t launch_fill(float *p, long long n, float v, hipStream_t st);
+hipError_t launch_src_amp(float *amp, int nsteps, int npts, const float *src,
+                          int ntsrc, const float *tab, int nsinc, int jt,
+                          int ntab, int t_off, float scale, hipStream_t st);
+hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
+                             const float *vel, float scale, hipStream_t st);
+hipError_t launch_transpose(float *dst, const float *src, int n0, int n1,
+                            hipStream_t st);
+
+}  // namespace born
