@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Born forward + exact adjoint throughput on one MI355X.
+
+Workload (the 256^3 Born fwd+adj pair at 10 m): 304^3 padded domain, 3 shots,
+128x128 receivers, 1500 steps at 1 ms.  One benchmark step runs, for every
+shot, the Born forward (data from the model) and the exact adjoint (image
+from those data), all inputs resident in HBM.  Work unit: a Born cell-step =
+one interior cell advanced one time step through the forward (2 wavefield
+updates) and the adjoint (2 wavefield updates); Mcell-updates/s = 4 x Born
+cell-steps/s.
+
+Prints one JSON line (see the keys below).  --gpus N with N > 1 is refused:
+the path shards by shot and the single-GPU number is the one measured.
+"""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "sc17-born_amd")
+
+HBM_PEAK_GBS = 8000.0
+FWD_BYTES_PER_CELL = 32      # s and r: read cur, read old, write new; vel; model
+ADJ_BYTES_PER_CELL = 36      # background recompute 12, adjoint state 12, vel 4, image r/w 8
+METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs FP32 plain"
+
+
+def load(name):
+    spec = importlib.util.spec_from_file_location("sc17born_" + name,
+                                                  os.path.join(PKG, name + ".py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[spec.name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(p, nthreads, target_s=12.0):
+    """The oracle (C restatement of the reference's cpu_prop.cc) on this host:
+    rtmForward + sourceProp + rtmAdjoint on the same domain, a bounded number
+    of steps, timed with all threads given."""
+    sys.path.insert(0, ROOT)
+    import oracle
+
+    def run(nt):
+        P = oracle.Prop(nthreads)
+        P.create_space(p.d, p.d, p.d, p.nx, p.ny, p.nz)
+        P.transfer_sinc_table_s(p.tab)
+        P.transfer_sinc_table_d(p.tab)
+        locs, vals = p.shots[0]
+        P.transfer_source_func(locs, vals)
+        P.transfer_vel_func1(p.V)
+        P.transfer_vel_func2(p.V)
+        rec = np.zeros((p.nrec, p.nt_big), np.float32)
+        P.transfer_receiver_func(p.rec_locs, rec)
+        a0 = np.zeros(p.nx * p.ny * p.nz, np.float32)
+        a1 = np.zeros_like(a0)
+        img = np.zeros_like(a0)
+        t0 = time.perf_counter()
+        P.rtm_forward(1, p.model, nt)
+        P.source_prop(a0, a1, 1, nt)
+        P.rtm_adjoint(a1, a0, img, nt)
+        return time.perf_counter() - t0
+
+    t4 = run(4)
+    nt = int(max(4, min(400, 4 * target_s / max(t4, 1e-3))))
+    t = run(nt)
+    cell_steps = p.interior_cells * nt
+    return {
+        "value": round(4 * cell_steps / t / 1e6, 2),
+        "unit": "Mcell-updates/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": "oracle rtmForward + sourceProp + rtmAdjoint (reference CPU semantics), "
+                  "%dx%dx%d domain, 1 shot, %d steps (%.1f s)" % (p.nx, p.ny, p.nz, nt, t),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--zchunk", type=int, default=0)
+    ap.add_argument("--budget-gb", type=float, default=0.0,
+                    help="HBM budget for the background history (0: free memory)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs one shot batch per GPU and "
+              "has no multi-GPU path; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    pyborn = load("pyborn")
+    configs = load("configs")
+    cfg = dict(configs.CONFIGS[args.config])
+    t_setup = time.perf_counter()
+    p = configs.make_problem(**cfg)
+    log("problem built in %.1f s: %dx%dx%d, %d shots, nt=%d"
+        % (time.perf_counter() - t_setup, p.nx, p.ny, p.nz, len(p.shots), p.nt))
+
+    dev = torch.device("cuda:0")
+    P = pyborn.Propagator(0)
+    P.create_space(p.d, p.d, p.d, p.nx, p.ny, p.nz)
+    P.transfer_sinc_table_s(p.tab)
+    P.transfer_sinc_table_d(p.tab)
+    P.transfer_vel_func1(p.V)
+    P.transfer_vel_func2(p.V)
+    P.transfer_receiver_func(p.rec_nx, p.rec_ny, p.rec_locs,
+                             np.zeros((p.nrec, p.nt_big), np.float32))
+    if args.zchunk:
+        P.set_zchunk(args.zchunk)
+    if args.budget_gb:
+        P.set_checkpoint_budget(int(args.budget_gb * 2 ** 30))
+    model = torch.from_numpy(p.model).to(dev)
+    traces = torch.zeros((p.nt_big, p.nrec), dtype=torch.float32, device=dev)
+    img = torch.zeros(p.nx * p.ny * p.nz, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(P.stream)
+    nt = p.nt
+    fwd_events = []
+    adj_events = []
+
+    def step(record):
+        img.zero_()
+        torch.cuda.synchronize()
+        for locs, vals in p.shots:
+            P.transfer_source_func(locs, vals)
+            traces.zero_()
+            torch.cuda.synchronize()
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record(stream)
+            P.dev_forward(1, nt, model, traces, scatter=0, keep_background=True)
+            e[1].record(stream)
+            P.dev_adjoint_exact(1, nt, traces, img, scatter=0)
+            e[2].record(stream)
+            if record:
+                fwd_events.append((e[0], e[1]))
+                adj_events.append((e[1], e[2]))
+        P.synchronize()
+
+    for i in range(args.warmup):
+        step(False)
+        log("warmup step %d done" % i)
+    torch.cuda.synchronize()
+    P.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(True)
+        log("timed step %d done" % i)
+    torch.cuda.synchronize()
+    P.synchronize()
+    elapsed = time.perf_counter() - t0
+    segs, recomputed = P.last_checkpoint_plan()
+    tuning = P.tuning()
+
+    nshots = len(p.shots)
+    cell_steps = p.interior_cells * nt * nshots * args.steps
+    value = 4 * cell_steps / elapsed / 1e6
+    fwd_ms = np.mean([a.elapsed_time(b) for a, b in fwd_events])
+    adj_ms = np.mean([a.elapsed_time(b) for a, b in adj_events])
+    # dominant kernel: the fused forward step (one launch per time step)
+    fwd_launch_s = fwd_ms / 1e3 / nt
+    fwd_alg_bytes = FWD_BYTES_PER_CELL * p.interior_cells
+    achieved = fwd_alg_bytes / fwd_launch_s / 1e9
+    alg_gbs = (FWD_BYTES_PER_CELL + ADJ_BYTES_PER_CELL) * cell_steps / elapsed / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "Born fwd+adj (exact adjoint), 256^3 model at 10 m -> %dx%dx%d "
+                        "domain, %d shots, %dx%d receivers, %d steps at 1 ms"
+                        % (p.nx, p.ny, p.nz, nshots, p.rec_nx, p.rec_ny, nt),
+            "global_batch": nshots,
+            "parallelism": "single GPU",
+            "born_cell_steps_per_s": round(cell_steps / elapsed, 1),
+            "algorithmic_GBps": round(alg_gbs, 1),
+            "algorithmic_frac_of_hbm": round(alg_gbs / HBM_PEAK_GBS, 4),
+            "forward_ms_per_shot": round(float(fwd_ms), 2),
+            "adjoint_ms_per_shot": round(float(adj_ms), 2),
+            "checkpoint_segments": segs,
+            "recomputed_steps": recomputed,
+            "tuning": {k: v for k, v in tuning.items() if v["ms"] >= 0},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "march_kernel<FwdOp<0>> (fused Born forward step)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_cell": FWD_BYTES_PER_CELL,
+            "avg_launch_us": round(fwd_launch_s * 1e6, 2),
+        },
+    }
+    P.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline(p, nthreads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -146,8 +146,19 @@ int born_set_checkpoint_budget(born_ctx *ctx, size_t bytes);
 /* segments and recomputed steps of the last exact adjoint */
 int born_last_checkpoint_plan(const born_ctx *ctx, int *segments,
                               int *recomputed_steps);
-/* z planes per block of the step kernels (0 = default) */
+/* z planes per block of the step kernels (0 = chosen by the autotuner) */
 int born_set_zchunk(born_ctx *ctx, int zchunk);
+
+/* Autotuner of the step kernels' launch shape (on by default): before the
+ * first launch of a step kernel on a new grid, every tile shape (rows per
+ * thread x wavefronts per block) and z chunk is timed on scratch fields and
+ * the fastest kept.  on = 0 keeps the default shape; on = 1 re-tunes. */
+int born_set_autotune(born_ctx *ctx, int on);
+/* The chosen launch shape of a step kernel: 0 Born forward, 1 Born forward
+ * (dV scatter), 2 source propagation, 3 reference adjoint, 4/5 background
+ * recompute, 6 adjoint state; ms = timed duration (-1 when not tuned). */
+int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
+                    int *waves_per_block, int *zchunk, float *ms);
 
 void *born_get_stream(const born_ctx *ctx);
 int born_synchronize(born_ctx *ctx);

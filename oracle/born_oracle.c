@@ -402,7 +402,10 @@ void orc_source_func(int n1, int n2, float o1, float o2, float o3, float d1,
         float xn = fx - i1;
         locs[i] = i1 + i2 * n1 + i3 * n1 * n2;
         float scale = expf(-zn * zn - xn * xn - yn * yn);
-        for (int it = 0; it < nw; it++) vals[nts * i + it + 4] = scale * wdev[it];
+        /* truncated to the trace (the reference writes past it when
+           nw > nts - 4, source_func_3d.cc:103-105) */
+        for (int it = 0; it < nw && it + 4 < nts; it++)
+          vals[nts * i + it + 4] = scale * wdev[it];
       }
     }
   }

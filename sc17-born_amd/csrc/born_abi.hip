@@ -40,14 +40,17 @@ struct born_ctx {
   int nbound = 40;
   float bcB = .0005f;
   float *d_bx = nullptr, *d_by = nullptr, *d_bz = nullptr;
-  int zchunk = 0;
+  int zchunk = 0;  // override of the tuned z chunk (0: tuned)
+  int autotune = 1;
+  born::Tune tune[7];
+  float tune_ms[7];
 
   int nsinc_s = 0, jts = 0, nsinc_d = 0, jtd = 0;
   float *d_tab_s = nullptr, *d_tab_d = nullptr;
 
   int npts_s = 0, nt_src = 0;
   float *d_src = nullptr;
-  born::SrcPts pts{};
+  int *d_pts = nullptr;  // int4 (x, y, z, 0) per source point
 
   float *d_vel1 = nullptr, *d_vel2 = nullptr;
 
@@ -136,7 +139,6 @@ Geometry geometry(const born_ctx *c) {
   g.nx = c->nx;
   g.ny = c->ny;
   g.nz = c->nz;
-  g.zchunk = c->zchunk;
   // reference tap indices: C0=0, CZk=1+3(k-1), CXk=2+3(k-1), CYk=3+3(k-1)
   // (base/cpu_prop.cc:6-18)
   g.taps.c0 = c->coeffs[0];
@@ -159,6 +161,59 @@ born::RecMap recmap(const born_ctx *c) {
   m.nrec = c->nrec;
   m.ntb = c->nt_rec;
   return m;
+}
+
+// step kernels that are tuned separately
+enum Kind { K_FWD0, K_FWD1, K_SRC, K_RADJ, K_SREC0, K_SREC1, K_LAM, K_NUM };
+
+born::SrcPts points(const born_ctx *c) { return {c->d_pts, c->npts_s}; }
+born::SrcPts no_points() { return {nullptr, 0}; }
+
+void reset_tuning(born_ctx *c) {
+  for (int k = 0; k < K_NUM; k++) {
+    c->tune[k] = {3, 64};  // 64x32 tiles of 2 rows per thread, 64 planes
+    c->tune_ms[k] = -1.f;
+  }
+}
+
+born::Tune tuning(const born_ctx *c, int kind) {
+  born::Tune t = c->tune[kind];
+  if (c->zchunk > 0) t.zchunk = c->zchunk;
+  return t;
+}
+
+// Autotuner: time every tile shape x z chunk for one step kernel and keep
+// the fastest.  run(tune) must launch the kernel on scratch buffers only.
+template <class F>
+int autotune(born_ctx *ctx, int kind, F run) {
+  if (!ctx->autotune || ctx->tune_ms[kind] >= 0.f) return BORN_OK;
+  static const int zcs[3] = {32, 64, 128};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  born::Tune bt = ctx->tune[kind];
+  for (int sh = 0; sh < born::kNumShapes; sh++)
+    for (int zi = 0; zi < 3; zi++) {
+      const born::Tune t{sh, ctx->zchunk > 0 ? ctx->zchunk : zcs[zi]};
+      if (ctx->zchunk > 0 && zi > 0) continue;
+      HIPCHK(run(t));  // warm
+      HIPCHK(hipEventRecord(e0, ctx->stream));
+      for (int r = 0; r < 3; r++) HIPCHK(run(t));
+      HIPCHK(hipEventRecord(e1, ctx->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bt = t;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->tune[kind] = bt;
+  ctx->tune_ms[kind] = best / 3.f;
+  return BORN_OK;
 }
 
 // Sponge factor per axis coordinate (cpu_prop.cc:141-165, 329-332).
@@ -346,6 +401,7 @@ int born_create(born_ctx **out, int device) {
   if (device < 0 || device >= ndev) return BORN_ERR_ARG;
   born_ctx *ctx = new born_ctx();
   ctx->device = device;
+  reset_tuning(ctx);
   if (hipSetDevice(device) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     (void)hipGetLastError();
@@ -360,7 +416,7 @@ int born_destroy(born_ctx *ctx) {
   if (!ctx) return BORN_ERR_ARG;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  void *bufs[] = {ctx->d_bx,    ctx->d_by,    ctx->d_bz,   ctx->d_tab_s,
+  void *bufs[] = {ctx->d_pts,   ctx->d_bx,    ctx->d_by,    ctx->d_bz,   ctx->d_tab_s,
                   ctx->d_tab_d, ctx->d_src,   ctx->d_vel1, ctx->d_vel2,
                   ctx->d_plane_slot, ctx->d_rmap, ctx->d_rnext, ctx->d_w[0],
                   ctx->d_w[1],  ctx->d_w[2],  ctx->d_w[3], ctx->d_img,
@@ -387,8 +443,9 @@ int born_create_space(born_ctx *ctx, float d1, float d2, float d3, float bc_a,
     return fail(ctx, BORN_ERR_ARG, "each axis needs at least 9 samples");
   if (!(d1 > 0) || !(d2 > 0) || !(d3 > 0))
     return fail(ctx, BORN_ERR_ARG, "sampling must be positive");
-  if ((long long)nx * ny * nz >= (1ll << 31))
-    return fail(ctx, BORN_ERR_UNSUPPORTED, "more than 2^31 cells");
+  if ((long long)nx * ny * nz > (1ll << 30))
+    return fail(ctx, BORN_ERR_UNSUPPORTED,
+                "more than 2^30 cells (fields are addressed with 32-bit byte offsets)");
   CHK(set_device(ctx));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   const bool resized = nx != ctx->nx || ny != ctx->ny || nz != ctx->nz;
@@ -421,6 +478,7 @@ int born_create_space(born_ctx *ctx, float d1, float d2, float d3, float bc_a,
     CHK(dev_alloc(ctx, &ctx->d_src, 0));
   }
   CHK(upload_sponge(ctx));
+  if (resized) reset_tuning(ctx);
   ctx->have_space = true;
   ctx->bg_version++;
   return BORN_OK;
@@ -478,19 +536,19 @@ int born_transfer_source_func(born_ctx *ctx, int npts, int nt_big, const int *lo
   if (npts > born::kMaxPts)
     return fail(ctx, BORN_ERR_UNSUPPORTED,
                 "at most " + std::to_string(born::kMaxPts) + " source points");
-  born::SrcPts pts{};
-  pts.n = npts;
+  std::vector<int> pts(4 * (size_t)std::max(npts, 1), 0);
   for (int i = 0; i < npts; i++)
-    if (!interior(ctx, locs[i], &pts.x[i], &pts.y[i], &pts.z[i]))
+    if (!interior(ctx, locs[i], &pts[4 * i], &pts[4 * i + 1], &pts[4 * i + 2]))
       return fail(ctx, BORN_ERR_UNSUPPORTED,
                   "source point " + std::to_string(i) + " outside the stencil interior");
   CHK(set_device(ctx));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   CHK(dev_alloc(ctx, &ctx->d_src, (size_t)std::max(npts, 1) * nt_big));
+  CHK(dev_alloc(ctx, &ctx->d_pts, pts.size()));
+  HIPCHK(hipMemcpy(ctx->d_pts, pts.data(), pts.size() * sizeof(int), hipMemcpyHostToDevice));
   if (npts)
     HIPCHK(hipMemcpy(ctx->d_src, vals, (size_t)npts * nt_big * sizeof(float),
                      hipMemcpyHostToDevice));
-  ctx->pts = pts;
   ctx->npts_s = npts;
   ctx->nt_src = nt_big;
   ctx->bg_version++;
@@ -582,15 +640,25 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   }
   ctx->hist_version = -1;
   ctx->jt_state = jt;
+  const Geometry geo = geometry(ctx);
+  const int kind = scatter == 0 ? K_FWD0 : K_FWD1;
+  {  // tune on the work fields (zeroed below), no traces written
+    born::FwdStep t{};
+    t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.r1 = ctx->d_w[3]; t.r0 = ctx->d_w[2];
+    t.vel = ctx->d_vel2; t.model = model; t.src = no_points(); t.rec = recmap(ctx);
+    t.tr = traces; t.tab = ctx->d_tab_d; t.extract = 0; t.scatter = scatter;
+    CHK(autotune(ctx, kind, [&](const born::Tune &tu) {
+      return born::launch_fwd_step(geo, tu, t, ctx->stream); }));
+  }
+  const born::Tune tu = tuning(ctx, kind);
   for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w[i], n));
   CHK(source_amps(ctx, nt, jt, 0, 1 / (float)jt));
-  const Geometry geo = geometry(ctx);
   float *s_old = ctx->d_w[0], *s_cur = ctx->d_w[1];
   float *r_old = ctx->d_w[2], *r_cur = ctx->d_w[3];
   born::FwdStep a{};
   a.vel = ctx->d_vel2;
   a.model = model;
-  a.src = ctx->pts;
+  a.src = points(ctx);
   a.rec = recmap(ctx);
   a.tr = traces;
   a.scatter = scatter;
@@ -605,7 +673,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     a.tab = ctx->d_tab_d + (long long)ii * 8;
     a.id = id;
     a.extract = id + 7 < ctx->nt_rec;
-    HIPCHK(born::launch_fwd_step(geo, a, ctx->stream));
+    HIPCHK(born::launch_fwd_step(geo, tu, a, ctx->stream));
     std::swap(s_old, s_cur);
     std::swap(r_old, r_cur);
   }
@@ -624,17 +692,25 @@ int born_dev_source_prop(born_ctx *ctx, int jts, int nt, float *p0, float *p1) {
   CHK(set_device(ctx));
   // cpu_prop.cc:108-122: _jt = jts scales, the table's jts indexes
   ctx->jt_state = jts;
-  CHK(source_amps(ctx, nt + 1, ctx->jts, 0, 1 / (float)jts));
   const Geometry geo = geometry(ctx);
+  CHK(ensure_work(ctx));
+  {  // tune on two work fields (unused by this loop)
+    born::SrcStep t{};
+    t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.vel = ctx->d_vel1; t.src = no_points();
+    CHK(autotune(ctx, K_SRC, [&](const born::Tune &tu) {
+      return born::launch_src_step(geo, tu, t, ctx->stream); }));
+  }
+  const born::Tune tu = tuning(ctx, K_SRC);
+  CHK(source_amps(ctx, nt + 1, ctx->jts, 0, 1 / (float)jts));
   float *a0 = p0, *a1 = p1;
   born::SrcStep s{};
   s.vel = ctx->d_vel1;
-  s.src = ctx->pts;
+  s.src = points(ctx);
   for (int it = 0; it <= nt; it++) {
     s.s1 = a1;
     s.s0 = a0;
     s.amp = ctx->d_amp + (long long)it * ctx->npts_s;
-    HIPCHK(born::launch_src_step(geo, s, ctx->stream));
+    HIPCHK(born::launch_src_step(geo, tu, s, ctx->stream));
     std::swap(a0, a1);
   }
   if (nt % 2 == 1)  // cpu_prop.cc:123-128
@@ -650,20 +726,31 @@ int born_dev_rtm_adjoint(born_ctx *ctx, int nt, float *p0, float *p1, float *img
     return fail(ctx, BORN_ERR_ARG, "bad rtm_adjoint arguments");
   CHK(set_device(ctx));
   CHK(ensure_work(ctx));
+  CHK(field_buf(ctx, &ctx->d_acc));
   const long long n = ctx->n;
+  const Geometry geo = geometry(ctx);
+  {  // tune on work fields and the accumulator (no trace read, image untouched)
+    born::RadjStep t{};
+    t.s1 = ctx->d_w[3]; t.s0 = ctx->d_w[2]; t.r1 = ctx->d_w[1]; t.r0 = ctx->d_w[0];
+    t.vel1 = ctx->d_vel1; t.vel2 = ctx->d_vel2; t.img = ctx->d_acc;
+    t.s_prop = 1; t.r_prop = 1; t.src = no_points(); t.amp_pre = ctx->d_vel1; t.pre = 0;
+    t.rec = recmap(ctx); t.tr = traces; t.tab = ctx->d_tab_d; t.inject = 0;
+    CHK(autotune(ctx, K_RADJ, [&](const born::Tune &tu) {
+      return born::launch_radj_step(geo, tu, t, ctx->stream); }));
+  }
+  const born::Tune tu = tuning(ctx, K_RADJ);
   CHK(zero(ctx, ctx->d_w[0], n));
   CHK(zero(ctx, ctx->d_w[1], n));
   // cpu_prop.cc:71-95 with _dir = -1 and _jt from the last sourceProp
   const float scale = -1 / (float)ctx->jt_state;
   if (nt > 0) CHK(source_amps(ctx, nt, ctx->jts, 1, scale));
-  const Geometry geo = geometry(ctx);
   float *s_old = p0, *s_cur = p1;
   float *r_old = ctx->d_w[0], *r_cur = ctx->d_w[1];
   born::RadjStep a{};
   a.vel1 = ctx->d_vel1;
   a.vel2 = ctx->d_vel2;
   a.img = img;
-  a.src = ctx->pts;
+  a.src = points(ctx);
   a.rec = recmap(ctx);
   a.tr = traces;
   a.sc = (float)-1 / (float)ctx->jt_state;
@@ -680,7 +767,7 @@ int born_dev_rtm_adjoint(born_ctx *ctx, int nt, float *p0, float *p1, float *img
     a.tab = ctx->d_tab_d + (long long)ii * 8;
     a.id = id;
     a.inject = id + 7 < ctx->nt_rec;
-    HIPCHK(born::launch_radj_step(geo, a, ctx->stream));
+    HIPCHK(born::launch_radj_step(geo, tu, a, ctx->stream));
     std::swap(s_old, s_cur);
     std::swap(r_old, r_cur);
   }
@@ -699,6 +786,21 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   const long long n = ctx->n;
   if (nt == 0) return BORN_OK;
   const Geometry geo = geometry(ctx);
+  const int skind = scatter == 0 ? K_SREC0 : K_SREC1;
+  {  // tune both kernels on work fields and the accumulator (zeroed below)
+    born::SrecStep t{};
+    t.s1 = ctx->d_w[3]; t.s0 = ctx->d_w[2]; t.vel = ctx->d_vel2; t.sig = ctx->d_acc;
+    t.src = no_points(); t.scatter = scatter;
+    CHK(autotune(ctx, skind, [&](const born::Tune &tu) {
+      return born::launch_srec_step(geo, tu, t, ctx->stream); }));
+    born::LamStep l{};
+    l.u1 = ctx->d_w[1]; l.u0 = ctx->d_w[0]; l.vel = ctx->d_vel2; l.sig = ctx->d_w[2];
+    l.acc = ctx->d_acc; l.rec = recmap(ctx); l.tr = traces; l.tab = ctx->d_tab_d;
+    l.inject = 0;
+    CHK(autotune(ctx, K_LAM, [&](const born::Tune &tu) {
+      return born::launch_lam_step(geo, tu, l, ctx->stream); }));
+  }
+  const born::Tune stu = tuning(ctx, skind), ltu = tuning(ctx, K_LAM);
   const bool cached = ctx->hist_version == ctx->bg_version &&
                       ctx->hist_scatter == scatter && ctx->hist_jt == jt &&
                       ctx->hist_nt == nt && ctx->seg_cap >= (size_t)nt;
@@ -726,7 +828,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   float *s_old = ctx->d_w[2], *s_cur = ctx->d_w[3];
   born::SrecStep sr{};
   sr.vel = ctx->d_vel2;
-  sr.src = ctx->pts;
+  sr.src = points(ctx);
   sr.scatter = scatter;
   const size_t Fb = (size_t)n * sizeof(float);
   int recomputed = 0;
@@ -746,7 +848,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
       sr.s0 = s_old;
       sr.sig = k >= klast ? ctx->d_seg + (long long)(k - klast) * n : nullptr;
       sr.amp = ctx->d_amp + (long long)k * ctx->npts_s;
-      HIPCHK(born::launch_srec_step(geo, sr, ctx->stream));
+      HIPCHK(born::launch_srec_step(geo, stu, sr, ctx->stream));
       std::swap(s_old, s_cur);
     }
   }
@@ -776,7 +878,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
         sr.s0 = s_old;
         sr.sig = ctx->d_seg + (long long)(k - k0) * n;
         sr.amp = ctx->d_amp + (long long)k * ctx->npts_s;
-        HIPCHK(born::launch_srec_step(geo, sr, ctx->stream));
+        HIPCHK(born::launch_srec_step(geo, stu, sr, ctx->stream));
         std::swap(s_old, s_cur);
         recomputed++;
       }
@@ -790,7 +892,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
       lm.tab = ctx->d_tab_d + (long long)ii * 8;
       lm.id = id;
       lm.inject = id + 7 < ctx->nt_rec;
-      HIPCHK(born::launch_lam_step(geo, lm, ctx->stream));
+      HIPCHK(born::launch_lam_step(geo, ltu, lm, ctx->stream));
       std::swap(u_old, u_cur);
     }
   }
@@ -920,6 +1022,24 @@ int born_last_checkpoint_plan(const born_ctx *ctx, int *segments, int *recompute
 int born_set_zchunk(born_ctx *ctx, int zchunk) {
   if (!ctx || zchunk < 0) return BORN_ERR_ARG;
   ctx->zchunk = zchunk;
+  return BORN_OK;
+}
+
+int born_set_autotune(born_ctx *ctx, int on) {
+  if (!ctx) return BORN_ERR_ARG;
+  ctx->autotune = on != 0;
+  if (on) reset_tuning(ctx);
+  return BORN_OK;
+}
+
+int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
+                    int *waves_per_block, int *zchunk, float *ms) {
+  if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
+  const born::Tune t = tuning(ctx, kernel);
+  if (rows_per_thread) *rows_per_thread = born::kShapes[t.shape].ry;
+  if (waves_per_block) *waves_per_block = born::kShapes[t.shape].wy;
+  if (zchunk) *zchunk = t.zchunk;
+  if (ms) *ms = ctx->tune_ms[kernel];
   return BORN_OK;
 }
 

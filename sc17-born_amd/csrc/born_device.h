@@ -1,30 +1,28 @@
 // Device-side building blocks of the Born propagator on CDNA4 (gfx950).
 //
-// One z-marching kernel template (march_kernel) advances one or two pressure
-// fields by one leapfrog step of the 8th-order constant-density acoustic
-// operator.  Each 256-thread block owns a 64 x 16 cell tile of the xy plane
-// (one 64-lane wavefront per x row, 4 rows per thread) and a chunk of z
-// planes.  The z neighbours live in a 9-deep register queue per row; the x/y
-// neighbours of the current plane come from an LDS tile with a 4-cell halo.
-// What the step does with the Laplacian is supplied by an "op" functor that
-// is inlined into the kernel: sponge, source injection, Born scatter,
-// receiver extraction/injection and imaging are all fused into the same pass
-// over HBM.
+// march_kernel advances one or two pressure fields by one leapfrog step of
+// the 8th-order constant-density acoustic operator.  A block of WY 64-lane
+// wavefronts owns a 64 x (RY*WY) cell tile of the xy plane (one wavefront per
+// x row group, RY rows per thread) and a chunk of z planes.  z neighbours
+// live in a 9-deep register queue per row; x/y neighbours of the current
+// plane come from an LDS tile with a 4-cell halo.  The loop is software
+// pipelined: the loads of plane z+1 (the queue's new leading plane, the LDS
+// halo and the op's pointwise inputs) are issued before plane z is computed,
+// so HBM latency overlaps the stencil arithmetic.
+//
+// What the step does with the Laplacian is supplied by an immutable "op"
+// functor inlined into the kernel: sponge, source injection, Born scatter,
+// receiver extraction/injection and imaging are fused into the same pass.
+// Ops are passed by value as kernel arguments and never written, so their
+// pointers stay in the global address space (global_load/store, not flat).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace born {
 
-constexpr int kTX = 64;              // x cells per tile (one wavefront)
-constexpr int kWY = 4;               // wavefronts per block
-constexpr int kRY = 4;               // y rows per thread
-constexpr int kTY = kWY * kRY;       // y rows per tile
-constexpr int kH = 4;                // stencil half-width
-constexpr int kLW = kTX + 2 * kH;    // LDS tile width
-constexpr int kLH = kTY + 2 * kH;    // LDS tile height
-constexpr int kThreads = kTX * kWY;  // threads per block
-constexpr int kMaxPts = 16;          // source points handled inside the step
+constexpr int kTX = 64;  // x cells per tile (one wavefront)
+constexpr int kH = 4;    // stencil half-width
 
 // Stencil taps in the reference's order and axis convention
 // (base/cpu_prop.cc:6-18, 233-248, 298-326): cx = fast axis, cy = middle,
@@ -38,13 +36,19 @@ struct Grid {
   int nx, ny, nz;
   int ntx, nty, ntz;  // tiles along x, y and z chunks
   int zc;             // planes per z chunk
-  long long n12;
+  uint32_t plane;     // nx*ny
 };
 
-// Up to kMaxPts point sources injected into the new level inside the step.
-struct PtList {
+// Per-axis sponge factors: bound[min(edge_x, edge_y, edge_z)] equals
+// min(bx[x], by[y], bz[z]) because the table grows with the edge distance.
+struct Sponge {
+  const float *bx, *by, *bz;
+};
+
+// Point sources: (x, y, z, 0) per point in device memory.
+struct Points {
+  const int4 *p;
   int n;
-  int x[kMaxPts], y[kMaxPts], z[kMaxPts];
 };
 
 // Receivers: every receiver plane z has a slot; rmap[slot][y][x] is the first
@@ -57,11 +61,15 @@ struct RecMap {
   int ntb;                // trace length (nt_big)
 };
 
-// Per-axis sponge factors: bound[min(edge_x, edge_y, edge_z)] equals
-// min(bx[x], by[y], bz[z]) because the table grows with the edge distance.
-struct Sponge {
-  const float *bx, *by, *bz;
-};
+// element at 32-bit element offset o (fields are < 2^30 cells)
+template <class T>
+__device__ __forceinline__ T &at(T *p, uint32_t o) {
+  return *(T *)((char *)p + (uint64_t)(o << 2));
+}
+template <class T>
+__device__ __forceinline__ const T &at(const T *p, uint32_t o) {
+  return *(const T *)((const char *)p + (uint64_t)(o << 2));
+}
 
 // The hardware hands block h to XCD h % 8.  Give every XCD a contiguous range
 // of logical tiles so that tiles sharing halo rows share an L2.
@@ -81,24 +89,64 @@ __device__ __forceinline__ float sinc8(const float *t, const float *v,
   return s;
 }
 
+// Uniform per-block mask of the source points inside the block's cells.
+__device__ __forceinline__ uint32_t points_in_block(const Points &pts, int x0,
+                                                   int x1, int y0, int y1,
+                                                   int z0, int z1) {
+  uint32_t m = 0;
+  for (int i = 0; i < pts.n; i++) {
+    const int4 q = pts.p[i];
+    if (q.x >= x0 && q.x < x1 && q.y >= y0 && q.y < y1 && q.z >= z0 && q.z < z1)
+      m |= 1u << i;
+  }
+  return m;
+}
+
+__device__ __forceinline__ uint32_t points_on_plane(const Points &pts,
+                                                    uint32_t mask, int z) {
+  uint32_t m = 0;
+  for (uint32_t b = mask; b; b &= b - 1) {
+    const int i = __builtin_ctz(b);
+    if (pts.p[i].z == z) m |= 1u << i;
+  }
+  return m;
+}
+
+// v plus, in list order, amp[i] of the points at (x, y) among plane mask m
+__device__ __forceinline__ float add_points(const Points &pts, uint32_t m, int x,
+                                            int y, float v, const float *amp) {
+  for (uint32_t b = m; b; b &= b - 1) {
+    const int i = __builtin_ctz(b);
+    const int4 q = pts.p[i];
+    if (q.x == x && q.y == y) v = __fadd_rn(v, amp[i]);
+  }
+  return v;
+}
+
 struct CellArgs {
   int x, y, z;
-  long long idx;
-  float b;  // sponge factor (1 outside the sponge)
+  uint32_t o;  // element offset of the cell
+  float b;     // sponge factor (1 outside the sponge)
 };
 
-// Generic z-marching leapfrog kernel.  Op supplies:
-//   static constexpr int NF;               stencil fields (1 or 2)
-//   static constexpr bool DAMP;            whether the sponge factor is used
-//   const float *cur(int f) const;         current level of field f
-//   void block_init(x0, x1, y0, y1, z0, z1);  uniform per-block setup
-//   void plane(int z);                     uniform per-plane setup
-//   void cell(const CellArgs&, const float* lap, const float* ctr);
-template <class Op>
-__global__ void __launch_bounds__(kThreads)
-    march_kernel(const Grid g, const Taps t, const Sponge sp, Op op) {
+// Generic z-marching leapfrog kernel.  Op supplies (all const):
+//   static constexpr int NF, NPRE; static constexpr bool DAMP;
+//   const float *cur(int f);                 current level of field f
+//   Blk block(x0, x1, y0, y1, z0, z1);       uniform per-block state
+//   Pln plane(const Blk&, int z);            uniform per-plane state
+//   void prefetch(float (&pre)[NPRE], uint32_t o);   pointwise inputs
+//   void cell(const Pln&, const CellArgs&, const float *lap,
+//             const float *ctr, const float (&pre)[NPRE]);
+template <class Op, int RY, int WY>
+__global__ void __launch_bounds__(kTX *WY)
+    march_kernel(const Grid g, const Taps t, const Sponge sp, const Op op) {
   constexpr int NF = Op::NF;
-  __shared__ float tile[NF][kLH][kLW];
+  constexpr int NP = Op::NPRE > 0 ? Op::NPRE : 1;
+  constexpr int TY = RY * WY;
+  constexpr int LW = kTX + 2 * kH;
+  constexpr int LH = TY + 2 * kH;
+  constexpr int NXH = 2 * kH * TY;  // x-halo cells of a plane
+  __shared__ float tile[NF][LH][LW];
 
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const int bx = L % g.ntx;
@@ -107,72 +155,117 @@ __global__ void __launch_bounds__(kThreads)
   const int bz = rest / g.nty;
   const int tx = (int)threadIdx.x % kTX;
   const int ty = (int)threadIdx.x / kTX;
-  const int x0 = kH + bx * kTX, y0 = kH + by * kTY;
+  // x tiles start at the row start so that a wavefront's row maps onto whole
+  // cache lines; the 4-cell shell at either end is masked
+  const int x0 = bx * kTX, y0 = kH + by * TY;
   const int z0 = kH + bz * g.zc;
   const int z1 = min(z0 + g.zc, g.nz - kH);
   const int x = x0 + tx;
-  const bool xin = x < g.nx - kH;
+  const bool xin = x >= kH && x < g.nx - kH;
   const int xc = min(x, g.nx - 1);
 
-  op.block_init(x0, min(x0 + kTX, g.nx - kH), y0, min(y0 + kTY, g.ny - kH),
-                z0, z1);
+  const auto blk = op.block(max(x0, kH), min(x0 + kTX, g.nx - kH), y0,
+                            min(y0 + TY, g.ny - kH), z0, z1);
 
-  int yy[kRY];
-  bool in[kRY];
-  long long row[kRY];
-  float bxy[kRY];
+  int yy[RY];
+  bool in[RY];
+  uint32_t row[RY];
+  float bxy[RY];
 #pragma unroll
-  for (int j = 0; j < kRY; j++) {
-    yy[j] = y0 + ty + kWY * j;
+  for (int j = 0; j < RY; j++) {
+    yy[j] = y0 + ty + WY * j;
     in[j] = xin && yy[j] < g.ny - kH;
     const int yc = min(yy[j], g.ny - 1);
-    row[j] = (long long)yc * g.nx + xc;
+    row[j] = (uint32_t)yc * g.nx + xc;
     bxy[j] = Op::DAMP ? fminf(sp.bx[xc], sp.by[yc]) : 1.f;
   }
 
-  // halo bookkeeping: 8 y-halo rows (two per thread) and 8 x-halo columns
-  // of the 16 tile rows (one cell for each of the first 128 threads)
-  const long long yha = (long long)(y0 - kH + ty) * g.nx + xc;
-  const long long yhb = (long long)min(y0 + kTY + ty, g.ny - 1) * g.nx + xc;
-  const int tid = (int)threadIdx.x;
-  const bool xh = tid < 2 * kH * kTY;
-  const int xh_r = (tid / (2 * kH)) % kTY;
-  const int xh_c = tid % (2 * kH);
-  const int xh_l = xh_c < kH ? xh_c : kTX + xh_c;
-  const long long xho = (long long)min(y0 + xh_r, g.ny - 1) * g.nx +
-                        min(x0 - kH + xh_l, g.nx - 1);
+  // LDS halo: 8 y-halo rows of 64 cells and 8 x-halo columns of TY rows,
+  // spread over the block's threads
+  constexpr int NYH = 2 * kH * kTX;          // y-halo cells of a plane
+  constexpr int NHALO = NYH + NXH;
+  constexpr int HPT = (NHALO + kTX * WY - 1) / (kTX * WY);  // per thread
+  uint32_t hoff[HPT];
+  short hr[HPT], hc[HPT];
+  bool hv[HPT];
+#pragma unroll
+  for (int k = 0; k < HPT; k++) {
+    const int h = (int)threadIdx.x + k * kTX * WY;
+    hv[k] = h < NHALO;
+    int lr, lc;
+    if (h < NYH) {  // rows 0..3 and TY+4..TY+7, columns 4..67
+      const int r = h / kTX;
+      lr = r < kH ? r : TY + r;
+      lc = kH + h % kTX;
+    } else {  // columns 0..3 and 68..71 of rows 4..TY+3
+      const int hh = min(h - NYH, NXH - 1);
+      const int c = hh % (2 * kH);
+      lr = kH + hh / (2 * kH);
+      lc = c < kH ? c : kTX + c;
+    }
+    hr[k] = (short)lr;
+    hc[k] = (short)lc;
+    const int gy = min(max(y0 - kH + lr, 0), g.ny - 1);
+    const int gx = min(max(x0 - kH + lc, 0), g.nx - 1);
+    hoff[k] = (uint32_t)gy * g.nx + gx;
+  }
 
-  float q[NF][kRY][2 * kH + 1];
+  float q[NF][RY][2 * kH + 1];
+  float lead[NF][RY];
+  float halo[NF][HPT];
+  float pre[RY][NP], pre_n[RY][NP];
 #pragma unroll
   for (int f = 0; f < NF; f++) {
     const float *p = op.cur(f);
 #pragma unroll
-    for (int j = 0; j < kRY; j++)
+    for (int j = 0; j < RY; j++) {
 #pragma unroll
       for (int k = 0; k < 2 * kH; k++)
-        q[f][j][k] = p[(long long)(z0 - kH + k) * g.n12 + row[j]];
+        q[f][j][k] = at(p, (uint32_t)(z0 - kH + k) * g.plane + row[j]);
+      lead[f][j] = at(p, (uint32_t)(z0 + kH) * g.plane + row[j]);
+    }
+#pragma unroll
+    for (int k = 0; k < HPT; k++)
+      halo[f][k] = hv[k] ? at(p, (uint32_t)z0 * g.plane + hoff[k]) : 0.f;
   }
+#pragma unroll
+  for (int j = 0; j < RY; j++) op.prefetch(pre[j], (uint32_t)z0 * g.plane + row[j]);
 
   for (int z = z0; z < z1; ++z) {
-    const long long zo = (long long)z * g.n12;
+    const uint32_t zo = (uint32_t)z * g.plane;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-      const float *p = op.cur(f);
 #pragma unroll
-      for (int j = 0; j < kRY; j++) q[f][j][2 * kH] = p[zo + kH * g.n12 + row[j]];
+      for (int j = 0; j < RY; j++) {
+        q[f][j][2 * kH] = lead[f][j];
+        tile[f][kH + ty + WY * j][kH + tx] = q[f][j][kH];
+      }
 #pragma unroll
-      for (int j = 0; j < kRY; j++) tile[f][kH + ty + kWY * j][kH + tx] = q[f][j][kH];
-      tile[f][ty][kH + tx] = p[zo + yha];
-      tile[f][kTY + kH + ty][kH + tx] = p[zo + yhb];
-      if (xh) tile[f][kH + xh_r][xh_l] = p[zo + xho];
+      for (int k = 0; k < HPT; k++)
+        if (hv[k]) tile[f][hr[k]][hc[k]] = halo[f][k];
+    }
+    // next plane's loads, in flight during this plane's compute
+    if (z + 1 < z1) {
+      const uint32_t zn = zo + g.plane;
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        const float *p = op.cur(f);
+#pragma unroll
+        for (int j = 0; j < RY; j++) lead[f][j] = at(p, zn + kH * g.plane + row[j]);
+#pragma unroll
+        for (int k = 0; k < HPT; k++)
+          if (hv[k]) halo[f][k] = at(p, zn + hoff[k]);
+      }
+#pragma unroll
+      for (int j = 0; j < RY; j++) op.prefetch(pre_n[j], zn + row[j]);
     }
     __syncthreads();
-    op.plane(z);
+    const auto pl = op.plane(blk, z);
     const float bzv = Op::DAMP ? sp.bz[z] : 1.f;
 #pragma unroll
-    for (int j = 0; j < kRY; j++) {
+    for (int j = 0; j < RY; j++) {
       if (!in[j]) continue;
-      const int lr = kH + ty + kWY * j, lc = kH + tx;
+      const int lr = kH + ty + WY * j, lc = kH + tx;
       float lap[NF], ctr[NF];
 #pragma unroll
       for (int f = 0; f < NF; f++) {
@@ -193,49 +286,22 @@ __global__ void __launch_bounds__(kThreads)
       c.x = x;
       c.y = yy[j];
       c.z = z;
-      c.idx = zo + row[j];
+      c.o = zo + row[j];
       c.b = fminf(bxy[j], bzv);
-      op.cell(c, lap, ctr);
+      op.cell(pl, c, lap, ctr, pre[j]);
     }
     __syncthreads();
 #pragma unroll
     for (int f = 0; f < NF; f++)
 #pragma unroll
-      for (int j = 0; j < kRY; j++)
+      for (int j = 0; j < RY; j++)
 #pragma unroll
         for (int k = 0; k < 2 * kH; k++) q[f][j][k] = q[f][j][k + 1];
+#pragma unroll
+    for (int j = 0; j < RY; j++)
+#pragma unroll
+      for (int k = 0; k < NP; k++) pre[j][k] = pre_n[j][k];
   }
 }
-
-// Uniform per-block source-point mask: bit i set when point i lies in the
-// block's cells.  pz_mask (per plane) narrows it to points on plane z.
-struct SrcInBlock {
-  uint32_t mask;
-  __device__ void init(const PtList &pts, int x0, int x1, int y0, int y1,
-                       int z0, int z1) {
-    mask = 0;
-    for (int i = 0; i < pts.n; i++)
-      if (pts.x[i] >= x0 && pts.x[i] < x1 && pts.y[i] >= y0 &&
-          pts.y[i] < y1 && pts.z[i] >= z0 && pts.z[i] < z1)
-        mask |= 1u << i;
-  }
-  __device__ uint32_t on_plane(const PtList &pts, int z) const {
-    uint32_t m = 0;
-    for (uint32_t b = mask; b; b &= b - 1) {
-      const int i = __builtin_ctz(b);
-      if (pts.z[i] == z) m |= 1u << i;
-    }
-    return m;
-  }
-  // sum, in list order, of amp[i] for the points at (x, y) among plane mask m
-  __device__ static float add(const PtList &pts, uint32_t m, int x, int y,
-                              float v, const float *amp) {
-    for (uint32_t b = m; b; b &= b - 1) {
-      const int i = __builtin_ctz(b);
-      if (pts.x[i] == x && pts.y[i] == y) v = __fadd_rn(v, amp[i]);
-    }
-    return v;
-  }
-};
 
 }  // namespace born

@@ -23,69 +23,78 @@
 
 namespace born {
 
-struct RecvCtx {
-  RecMap rm;
-  int slot;  // receiver slot of the current plane (-1: none)
-  __device__ void plane(int z) { slot = rm.plane_slot[z]; }
-  __device__ int first(int x, int y, int nx, int ny) const {
-    return rm.rmap[((long long)slot * ny + y) * nx + x];
-  }
+struct NoBlk {};
+struct RecPln {
+  int slot;
 };
+struct SrcPln {
+  uint32_t pm;
+  int slot;
+};
+
+__device__ __forceinline__ int rec_first(const RecMap &rm, int slot, int x, int y,
+                                         int nx, int ny) {
+  return rm.rmap[((long long)slot * ny + y) * nx + x];
+}
 
 // ---------------------------------------------------------------- forward
 template <int SCATTER>
 struct FwdOp {
-  static constexpr int NF = 2;
+  static constexpr int NF = 2, NPRE = 4;
   static constexpr bool DAMP = true;
-  const float *s1, *r1;
-  float *s0, *r0;
-  const float *vel, *model;
-  float *hist;        // optional scatter-operand history (nullptr: none)
-  PtList pts;
-  const float *amp;   // source amplitudes of this step [npts]
-  RecvCtx rc;
-  float *tr;          // traces, time-major [ntb][nrec]
-  const float *tab;   // receiver sinc row of this step [8]
-  int id;             // trace time index of this step
-  int extract;        // id + 7 < ntb
-  int nx, ny;
-  SrcInBlock sb;
-  uint32_t pm;
+  const float *__restrict__ s1;
+  const float *__restrict__ r1;
+  float *__restrict__ s0;
+  float *__restrict__ r0;
+  const float *__restrict__ vel;
+  const float *__restrict__ model;
+  float *__restrict__ hist;  // optional scatter-operand history
+  Points pts;
+  const float *__restrict__ amp;  // source amplitudes of this step [npts]
+  RecMap rm;
+  float *__restrict__ tr;  // traces, time-major [ntb][nrec]
+  const float *__restrict__ tab;
+  int id, extract, nx, ny;
 
   __device__ const float *cur(int f) const { return f == 0 ? s1 : r1; }
-  __device__ void block_init(int x0, int x1, int y0, int y1, int z0, int z1) {
-    sb.init(pts, x0, x1, y0, y1, z0, z1);
+  __device__ uint32_t block(int x0, int x1, int y0, int y1, int z0, int z1) const {
+    return points_in_block(pts, x0, x1, y0, y1, z0, z1);
   }
-  __device__ void plane(int z) {
-    pm = sb.on_plane(pts, z);
-    rc.plane(z);
+  __device__ SrcPln plane(uint32_t blk, int z) const {
+    return {points_on_plane(pts, blk, z), rm.plane_slot[z]};
   }
-  __device__ void cell(const CellArgs &c, const float *lap, const float *ctr) {
-    const float v = vel[c.idx];
-    const float b = c.b;
-    float sn = b * (v * lap[0] + ctr[0] + ctr[0] - b * s0[c.idx]);
-    if (pm) sn = SrcInBlock::add(pts, pm, c.x, c.y, sn, amp);
-    float rn = b * (v * lap[1] + ctr[1] + ctr[1] - b * r0[c.idx]);
+  __device__ void prefetch(float (&p)[NPRE], uint32_t o) const {
+    p[0] = at(s0, o);
+    p[1] = at(r0, o);
+    p[2] = at(vel, o);
+    p[3] = at(model, o);
+  }
+  __device__ void cell(const SrcPln &pl, const CellArgs &c, const float *lap,
+                       const float *ctr, const float (&p)[NPRE]) const {
+    const float v = p[2], b = c.b;
+    float sn = b * (v * lap[0] + ctr[0] + ctr[0] - b * p[0]);
+    if (pl.pm) sn = add_points(pts, pl.pm, c.x, c.y, sn, amp);
+    float rn = b * (v * lap[1] + ctr[1] + ctr[1] - b * p[1]);
     float sig;
     if (SCATTER == 0) {
       // the reference's double literal: evaluated in double, rounded once
       // (cpu_prop.cc:220)
-      rn = (float)((double)rn + .000001 * (double)sn * (double)model[c.idx]);
+      rn = (float)((double)rn + .000001 * (double)sn * (double)p[3]);
       sig = sn;
     } else {
       sig = b * lap[0];
-      rn = rn + model[c.idx] * sig;
+      rn = rn + p[3] * sig;
     }
-    s0[c.idx] = sn;
-    r0[c.idx] = rn;
-    if (hist) hist[c.idx] = sig;
-    if (rc.slot >= 0 && extract) {
-      for (int i = rc.first(c.x, c.y, nx, ny); i >= 0; i = rc.rm.rnext[i]) {
-        float *p = tr + (long long)id * rc.rm.nrec + i;
+    at(s0, c.o) = sn;
+    at(r0, c.o) = rn;
+    if (hist) at(hist, c.o) = sig;
+    if (pl.slot >= 0 && extract) {
+      for (int i = rec_first(rm, pl.slot, c.x, c.y, nx, ny); i >= 0; i = rm.rnext[i]) {
+        float *q = tr + (long long)id * rm.nrec + i;
 #pragma unroll
         for (int k = 0; k < 8; k++)
-          p[(long long)k * rc.rm.nrec] =
-              __fadd_rn(p[(long long)k * rc.rm.nrec], __fmul_rn(rn, tab[k]));
+          q[(long long)k * rm.nrec] =
+              __fadd_rn(q[(long long)k * rm.nrec], __fmul_rn(rn, tab[k]));
       }
     }
   }
@@ -93,106 +102,121 @@ struct FwdOp {
 
 // ------------------------------------------------------------- sourceProp
 struct SrcOp {
-  static constexpr int NF = 1;
+  static constexpr int NF = 1, NPRE = 2;
   static constexpr bool DAMP = false;
-  const float *s1;
-  float *s0;
-  const float *vel;
-  PtList pts;
-  const float *amp;
-  SrcInBlock sb;
-  uint32_t pm;
+  const float *__restrict__ s1;
+  float *__restrict__ s0;
+  const float *__restrict__ vel;
+  Points pts;
+  const float *__restrict__ amp;
   __device__ const float *cur(int) const { return s1; }
-  __device__ void block_init(int x0, int x1, int y0, int y1, int z0, int z1) {
-    sb.init(pts, x0, x1, y0, y1, z0, z1);
+  __device__ uint32_t block(int x0, int x1, int y0, int y1, int z0, int z1) const {
+    return points_in_block(pts, x0, x1, y0, y1, z0, z1);
   }
-  __device__ void plane(int z) { pm = sb.on_plane(pts, z); }
-  __device__ void cell(const CellArgs &c, const float *lap, const float *ctr) {
-    float sn = vel[c.idx] * lap[0] + ctr[0] + ctr[0] - s0[c.idx];
-    if (pm) sn = SrcInBlock::add(pts, pm, c.x, c.y, sn, amp);
-    s0[c.idx] = sn;
+  __device__ uint32_t plane(uint32_t blk, int z) const {
+    return points_on_plane(pts, blk, z);
+  }
+  __device__ void prefetch(float (&p)[NPRE], uint32_t o) const {
+    p[0] = at(s0, o);
+    p[1] = at(vel, o);
+  }
+  __device__ void cell(uint32_t pm, const CellArgs &c, const float *lap, const float *ctr,
+                       const float (&p)[NPRE]) const {
+    float sn = p[1] * lap[0] + ctr[0] + ctr[0] - p[0];
+    if (pm) sn = add_points(pts, pm, c.x, c.y, sn, amp);
+    at(s0, c.o) = sn;
   }
 };
 
 // ------------------------------------------------------ reference adjoint
 struct RadjOp {
-  static constexpr int NF = 2;
+  static constexpr int NF = 2, NPRE = 5;
   static constexpr bool DAMP = true;
-  const float *s1, *r1;
-  float *s0, *r0;
-  const float *vel1, *vel2;
-  float *img;
+  const float *__restrict__ s1;
+  const float *__restrict__ r1;
+  float *__restrict__ s0;
+  float *__restrict__ r0;
+  const float *__restrict__ vel1;
+  const float *__restrict__ vel2;
+  float *__restrict__ img;
   int s_prop, r_prop;
   // source amplitude injected into the current level by the previous
   // iteration (cpu_prop.cc:89-92), applied here to the old level it became
-  PtList pts;
-  const float *amp_pre;
+  Points pts;
+  const float *__restrict__ amp_pre;
   int pre;
-  RecvCtx rc;
-  const float *tr;    // traces, time-major [ntb][nrec]
-  const float *tab;   // receiver sinc row [8]
-  float sc;           // (float)dir / (float)jt
-  int id, inject;
-  int nx, ny;
-  SrcInBlock sb;
-  uint32_t pm;
+  RecMap rm;
+  const float *__restrict__ tr;
+  const float *__restrict__ tab;
+  float sc;  // (float)dir / (float)jt
+  int id, inject, nx, ny;
 
   __device__ const float *cur(int f) const { return f == 0 ? s1 : r1; }
-  __device__ void block_init(int x0, int x1, int y0, int y1, int z0, int z1) {
-    sb.init(pts, x0, x1, y0, y1, z0, z1);
+  __device__ uint32_t block(int x0, int x1, int y0, int y1, int z0, int z1) const {
+    return pre ? points_in_block(pts, x0, x1, y0, y1, z0, z1) : 0u;
   }
-  __device__ void plane(int z) {
-    pm = pre ? sb.on_plane(pts, z) : 0u;
-    rc.plane(z);
+  __device__ SrcPln plane(uint32_t blk, int z) const {
+    return {points_on_plane(pts, blk, z), rm.plane_slot[z]};
   }
-  __device__ void cell(const CellArgs &c, const float *lap, const float *ctr) {
+  __device__ void prefetch(float (&p)[NPRE], uint32_t o) const {
+    p[0] = at(s0, o);
+    p[1] = at(r0, o);
+    p[2] = at(vel1, o);
+    p[3] = at(vel2, o);
+    p[4] = at(img, o);
+  }
+  __device__ void cell(const SrcPln &pl, const CellArgs &c, const float *lap,
+                       const float *ctr, const float (&p)[NPRE]) const {
     float sn;
     if (s_prop) {
-      float so = s0[c.idx];
-      if (pm) so = SrcInBlock::add(pts, pm, c.x, c.y, so, amp_pre);
-      sn = vel1[c.idx] * lap[0] + ctr[0] + ctr[0] - so;
-      s0[c.idx] = sn;
+      float so = p[0];
+      if (pl.pm) so = add_points(pts, pl.pm, c.x, c.y, so, amp_pre);
+      sn = p[2] * lap[0] + ctr[0] + ctr[0] - so;
+      at(s0, c.o) = sn;
     } else {
-      sn = s0[c.idx];
+      sn = p[0];
     }
     const float b = c.b;
-    const float ro = r0[c.idx];
-    float rn = r_prop ? b * (vel2[c.idx] * lap[1] + ctr[1] + ctr[1] - b * ro)
-                      : b * (b * ro);
-    if (rc.slot >= 0 && inject) {
-      for (int i = rc.first(c.x, c.y, nx, ny); i >= 0; i = rc.rm.rnext[i])
-        rn = __fadd_rn(rn, __fmul_rn(sc, sinc8(tab, tr + (long long)id * rc.rm.nrec + i,
-                                               rc.rm.nrec)));
+    float rn = r_prop ? b * (p[3] * lap[1] + ctr[1] + ctr[1] - b * p[1]) : b * (b * p[1]);
+    if (pl.slot >= 0 && inject) {
+      for (int i = rec_first(rm, pl.slot, c.x, c.y, nx, ny); i >= 0; i = rm.rnext[i])
+        rn = __fadd_rn(rn, __fmul_rn(sc, sinc8(tab, tr + (long long)id * rm.nrec + i,
+                                               rm.nrec)));
     }
-    r0[c.idx] = rn;
-    img[c.idx] += sn * rn;
+    at(r0, c.o) = rn;
+    at(img, c.o) = p[4] + sn * rn;
   }
 };
 
 // ------------------------------------- exact adjoint: background recompute
 template <int SCATTER>
 struct SrecOp {
-  static constexpr int NF = 1;
+  static constexpr int NF = 1, NPRE = 2;
   static constexpr bool DAMP = true;
-  const float *s1;
-  float *s0;
-  const float *vel;
-  float *sig;  // scatter operand of this step
-  PtList pts;
-  const float *amp;
-  SrcInBlock sb;
-  uint32_t pm;
+  const float *__restrict__ s1;
+  float *__restrict__ s0;
+  const float *__restrict__ vel;
+  float *__restrict__ sig;  // scatter operand of this step (nullptr: none)
+  Points pts;
+  const float *__restrict__ amp;
   __device__ const float *cur(int) const { return s1; }
-  __device__ void block_init(int x0, int x1, int y0, int y1, int z0, int z1) {
-    sb.init(pts, x0, x1, y0, y1, z0, z1);
+  __device__ uint32_t block(int x0, int x1, int y0, int y1, int z0, int z1) const {
+    return points_in_block(pts, x0, x1, y0, y1, z0, z1);
   }
-  __device__ void plane(int z) { pm = sb.on_plane(pts, z); }
-  __device__ void cell(const CellArgs &c, const float *lap, const float *ctr) {
+  __device__ uint32_t plane(uint32_t blk, int z) const {
+    return points_on_plane(pts, blk, z);
+  }
+  __device__ void prefetch(float (&p)[NPRE], uint32_t o) const {
+    p[0] = at(s0, o);
+    p[1] = at(vel, o);
+  }
+  __device__ void cell(uint32_t pm, const CellArgs &c, const float *lap, const float *ctr,
+                       const float (&p)[NPRE]) const {
     const float b = c.b;
-    float sn = b * (vel[c.idx] * lap[0] + ctr[0] + ctr[0] - b * s0[c.idx]);
-    if (pm) sn = SrcInBlock::add(pts, pm, c.x, c.y, sn, amp);
-    s0[c.idx] = sn;
-    if (sig) sig[c.idx] = SCATTER == 0 ? sn : b * lap[0];
+    float sn = b * (p[1] * lap[0] + ctr[0] + ctr[0] - b * p[0]);
+    if (pm) sn = add_points(pts, pm, c.x, c.y, sn, amp);
+    at(s0, c.o) = sn;
+    if (sig) at(sig, c.o) = SCATTER == 0 ? sn : b * lap[0];
   }
 };
 
@@ -200,33 +224,38 @@ struct SrecOp {
 // nu = V B lambda obeys the forward recursion with V B-weighted receiver
 // injection; the image accumulates sigma * nu and is divided by V B at the end.
 struct LamOp {
-  static constexpr int NF = 1;
+  static constexpr int NF = 1, NPRE = 4;
   static constexpr bool DAMP = true;
-  const float *u1;
-  float *u0;
-  const float *vel;
-  const float *sig;
-  float *acc;
-  RecvCtx rc;
-  const float *tr;
-  const float *tab;
-  int id, inject;
-  int nx, ny;
+  const float *__restrict__ u1;
+  float *__restrict__ u0;
+  const float *__restrict__ vel;
+  const float *__restrict__ sig;
+  float *__restrict__ acc;
+  RecMap rm;
+  const float *__restrict__ tr;
+  const float *__restrict__ tab;
+  int id, inject, nx, ny;
   __device__ const float *cur(int) const { return u1; }
-  __device__ void block_init(int, int, int, int, int, int) {}
-  __device__ void plane(int z) { rc.plane(z); }
-  __device__ void cell(const CellArgs &c, const float *lap, const float *ctr) {
-    const float b = c.b;
-    const float v = vel[c.idx];
-    float un = b * (v * lap[0] + ctr[0] + ctr[0] - b * u0[c.idx]);
-    if (rc.slot >= 0 && inject) {
+  __device__ int block(int, int, int, int, int, int) const { return 0; }
+  __device__ int plane(int, int z) const { return rm.plane_slot[z]; }
+  __device__ void prefetch(float (&p)[NPRE], uint32_t o) const {
+    p[0] = at(u0, o);
+    p[1] = at(vel, o);
+    p[2] = at(sig, o);
+    p[3] = at(acc, o);
+  }
+  __device__ void cell(int slot, const CellArgs &c, const float *lap, const float *ctr,
+                       const float (&p)[NPRE]) const {
+    const float b = c.b, v = p[1];
+    float un = b * (v * lap[0] + ctr[0] + ctr[0] - b * p[0]);
+    if (slot >= 0 && inject) {
       float g = 0.f;
-      for (int i = rc.first(c.x, c.y, nx, ny); i >= 0; i = rc.rm.rnext[i])
-        g += sinc8(tab, tr + (long long)id * rc.rm.nrec + i, rc.rm.nrec);
+      for (int i = rec_first(rm, slot, c.x, c.y, nx, ny); i >= 0; i = rm.rnext[i])
+        g += sinc8(tab, tr + (long long)id * rm.nrec + i, rm.nrec);
       un += v * b * g;
     }
-    u0[c.idx] = un;
-    acc[c.idx] += sig[c.idx] * un;
+    at(u0, c.o) = un;
+    at(acc, c.o) = p[3] + p[2] * un;
   }
 };
 
@@ -241,10 +270,9 @@ __global__ void fill_kernel(float *p, long long n, float v) {
 // amp[s*npts + i] = scale * sinc8(tab[ii], src[i][id..id+7]) with
 // tt = s + t_off, id = tt / jt, ii = tt - id*jt; zero when id + 7 >= ntsrc
 // (the guard of cpu_prop.cc:167).
-__global__ void src_amp_kernel(float *amp, int nsteps, int npts,
-                               const float *src, int ntsrc, const float *tab,
-                               int nsinc, int jt, int ntab, int t_off,
-                               float scale) {
+__global__ void src_amp_kernel(float *amp, int nsteps, int npts, const float *src,
+                               int ntsrc, const float *tab, int nsinc, int jt, int ntab,
+                               int t_off, float scale) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nsteps * npts) return;
   const int s = t / npts, i = t % npts;
@@ -253,22 +281,19 @@ __global__ void src_amp_kernel(float *amp, int nsteps, int npts,
   const int ii = tt - id * jt;
   float a = 0.f;
   if (id + 7 < ntsrc && ii < ntab)
-    a = __fmul_rn(scale, sinc8(tab + (long long)ii * nsinc,
-                               src + (long long)i * ntsrc + id, 1));
+    a = __fmul_rn(scale, sinc8(tab + (long long)ii * nsinc, src + (long long)i * ntsrc + id, 1));
   amp[t] = a;
 }
 
 // out[i] += scale * acc[i] / (vel[i] * b[i]) on the interior.
-__global__ void adj_finish_kernel(float *out, const float *acc,
-                                  const float *vel, const Sponge sp, int nx,
-                                  int ny, int nz, float scale) {
+__global__ void adj_finish_kernel(float *out, const float *acc, const float *vel,
+                                  const Sponge sp, int nx, int ny, int nz, float scale) {
   const long long n12 = (long long)nx * ny;
   const long long n = n12 * nz;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / n12);
-    if (x < kH || x >= nx - kH || y < kH || y >= ny - kH || z < kH ||
-        z >= nz - kH)
+    if (x < kH || x >= nx - kH || y < kH || y >= ny - kH || z < kH || z >= nz - kH)
       continue;
     const float b = fminf(fminf(sp.bx[x], sp.by[y]), sp.bz[z]);
     out[i] += scale * acc[i] / (vel[i] * b);
@@ -292,95 +317,108 @@ __global__ void transpose_kernel(float *dst, const float *src, int n0, int n1) {
 
 // ------------------------------------------------------------- launchers
 
-static Grid make_grid(const Geometry &geo) {
+const TileShape kShapes[kNumShapes] = {{2, 4}, {4, 4}, {2, 8}, {2, 16}, {4, 8}, {3, 8}};
+
+static Grid make_grid(const Geometry &geo, const TileShape &ts, int zc) {
   Grid g;
   g.nx = geo.nx;
   g.ny = geo.ny;
   g.nz = geo.nz;
-  g.n12 = (long long)geo.nx * geo.ny;
-  g.ntx = (geo.nx - 2 * kH + kTX - 1) / kTX;
-  g.nty = (geo.ny - 2 * kH + kTY - 1) / kTY;
-  g.zc = geo.zchunk > 0 ? geo.zchunk : 32;
+  g.plane = (uint32_t)geo.nx * geo.ny;
+  const int ty = ts.ry * ts.wy;
+  g.ntx = (geo.nx - kH + kTX - 1) / kTX;
+  g.nty = (geo.ny - 2 * kH + ty - 1) / ty;
+  g.zc = zc > 0 ? zc : 32;
   g.ntz = (geo.nz - 2 * kH + g.zc - 1) / g.zc;
   return g;
 }
 
-template <class Op>
-static hipError_t launch(const Geometry &geo, const Op &op, hipStream_t st) {
-  const Grid g = make_grid(geo);
+template <class Op, int RY, int WY>
+static hipError_t launch_shape(const Geometry &geo, int zc, const Op &op, hipStream_t st) {
+  const Grid g = make_grid(geo, {RY, WY}, zc);
   const int nb = g.ntx * g.nty * g.ntz;
-  hipLaunchKernelGGL((march_kernel<Op>), dim3(nb), dim3(kThreads), 0, st, g,
+  hipLaunchKernelGGL((march_kernel<Op, RY, WY>), dim3(nb), dim3(kTX * WY), 0, st, g,
                      geo.taps, geo.sponge, op);
   return hipGetLastError();
 }
 
-static PtList to_pts(const SrcPts &s) {
-  PtList p;
-  p.n = s.n;
-  for (int i = 0; i < kMaxPts; i++) {
-    p.x[i] = i < s.n ? s.x[i] : -1;
-    p.y[i] = i < s.n ? s.y[i] : -1;
-    p.z[i] = i < s.n ? s.z[i] : -1;
+template <class Op>
+static hipError_t launch(const Geometry &geo, const Tune &tu, const Op &op, hipStream_t st) {
+  switch (tu.shape) {
+    case 0: return launch_shape<Op, 2, 4>(geo, tu.zchunk, op, st);
+    case 1: return launch_shape<Op, 4, 4>(geo, tu.zchunk, op, st);
+    case 2: return launch_shape<Op, 2, 8>(geo, tu.zchunk, op, st);
+    case 3: return launch_shape<Op, 2, 16>(geo, tu.zchunk, op, st);
+    case 4: return launch_shape<Op, 4, 8>(geo, tu.zchunk, op, st);
+    case 5: return launch_shape<Op, 3, 8>(geo, tu.zchunk, op, st);
+    default: return hipErrorInvalidValue;
   }
-  return p;
 }
 
-hipError_t launch_fwd_step(const Geometry &geo, const FwdStep &a, hipStream_t st) {
-  if (a.scatter == 0) {
-    FwdOp<0> op{};
+static Points to_points(const SrcPts &s) { return {(const int4 *)s.p, s.n}; }
+
+hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a,
+                           hipStream_t st) {
+  auto fill = [&](auto &op) {
     op.s1 = a.s1; op.r1 = a.r1; op.s0 = a.s0; op.r0 = a.r0;
     op.vel = a.vel; op.model = a.model; op.hist = a.hist;
-    op.pts = to_pts(a.src); op.amp = a.amp;
-    op.rc.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
+    op.pts = to_points(a.src); op.amp = a.amp;
+    op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
     op.extract = a.extract; op.nx = geo.nx; op.ny = geo.ny;
-    return launch(geo, op, st);
+  };
+  if (a.scatter == 0) {
+    FwdOp<0> op{};
+    fill(op);
+    return launch(geo, tu, op, st);
   }
   FwdOp<1> op{};
-  op.s1 = a.s1; op.r1 = a.r1; op.s0 = a.s0; op.r0 = a.r0;
-  op.vel = a.vel; op.model = a.model; op.hist = a.hist;
-  op.pts = to_pts(a.src); op.amp = a.amp;
-  op.rc.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
-  op.extract = a.extract; op.nx = geo.nx; op.ny = geo.ny;
-  return launch(geo, op, st);
+  fill(op);
+  return launch(geo, tu, op, st);
 }
 
-hipError_t launch_src_step(const Geometry &geo, const SrcStep &a, hipStream_t st) {
+hipError_t launch_src_step(const Geometry &geo, const Tune &tu, const SrcStep &a,
+                           hipStream_t st) {
   SrcOp op{};
   op.s1 = a.s1; op.s0 = a.s0; op.vel = a.vel;
-  op.pts = to_pts(a.src); op.amp = a.amp;
-  return launch(geo, op, st);
+  op.pts = to_points(a.src); op.amp = a.amp;
+  return launch(geo, tu, op, st);
 }
 
-hipError_t launch_radj_step(const Geometry &geo, const RadjStep &a, hipStream_t st) {
+hipError_t launch_radj_step(const Geometry &geo, const Tune &tu, const RadjStep &a,
+                            hipStream_t st) {
   RadjOp op{};
   op.s1 = a.s1; op.r1 = a.r1; op.s0 = a.s0; op.r0 = a.r0;
   op.vel1 = a.vel1; op.vel2 = a.vel2; op.img = a.img;
   op.s_prop = a.s_prop; op.r_prop = a.r_prop;
-  op.pts = to_pts(a.src); op.amp_pre = a.amp_pre; op.pre = a.pre;
-  op.rc.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.sc = a.sc;
+  op.pts = to_points(a.src); op.amp_pre = a.amp_pre; op.pre = a.pre;
+  op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.sc = a.sc;
   op.id = a.id; op.inject = a.inject; op.nx = geo.nx; op.ny = geo.ny;
-  return launch(geo, op, st);
+  return launch(geo, tu, op, st);
 }
 
-hipError_t launch_srec_step(const Geometry &geo, const SrecStep &a, hipStream_t st) {
+hipError_t launch_srec_step(const Geometry &geo, const Tune &tu, const SrecStep &a,
+                            hipStream_t st) {
+  auto fill = [&](auto &op) {
+    op.s1 = a.s1; op.s0 = a.s0; op.vel = a.vel; op.sig = a.sig;
+    op.pts = to_points(a.src); op.amp = a.amp;
+  };
   if (a.scatter == 0) {
     SrecOp<0> op{};
-    op.s1 = a.s1; op.s0 = a.s0; op.vel = a.vel; op.sig = a.sig;
-    op.pts = to_pts(a.src); op.amp = a.amp;
-    return launch(geo, op, st);
+    fill(op);
+    return launch(geo, tu, op, st);
   }
   SrecOp<1> op{};
-  op.s1 = a.s1; op.s0 = a.s0; op.vel = a.vel; op.sig = a.sig;
-  op.pts = to_pts(a.src); op.amp = a.amp;
-  return launch(geo, op, st);
+  fill(op);
+  return launch(geo, tu, op, st);
 }
 
-hipError_t launch_lam_step(const Geometry &geo, const LamStep &a, hipStream_t st) {
+hipError_t launch_lam_step(const Geometry &geo, const Tune &tu, const LamStep &a,
+                           hipStream_t st) {
   LamOp op{};
   op.u1 = a.u1; op.u0 = a.u0; op.vel = a.vel; op.sig = a.sig; op.acc = a.acc;
-  op.rc.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
+  op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
   op.inject = a.inject; op.nx = geo.nx; op.ny = geo.ny;
-  return launch(geo, op, st);
+  return launch(geo, tu, op, st);
 }
 
 hipError_t launch_fill(float *p, long long n, float v, hipStream_t st) {
@@ -391,14 +429,13 @@ hipError_t launch_fill(float *p, long long n, float v, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_src_amp(float *amp, int nsteps, int npts, const float *src,
-                          int ntsrc, const float *tab, int nsinc, int jt,
-                          int ntab, int t_off, float scale, hipStream_t st) {
+hipError_t launch_src_amp(float *amp, int nsteps, int npts, const float *src, int ntsrc,
+                          const float *tab, int nsinc, int jt, int ntab, int t_off,
+                          float scale, hipStream_t st) {
   const int n = nsteps * npts;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(src_amp_kernel, dim3((n + 255) / 256), dim3(256), 0, st,
-                     amp, nsteps, npts, src, ntsrc, tab, nsinc, jt, ntab,
-                     t_off, scale);
+  hipLaunchKernelGGL(src_amp_kernel, dim3((n + 255) / 256), dim3(256), 0, st, amp, nsteps,
+                     npts, src, ntsrc, tab, nsinc, jt, ntab, t_off, scale);
   return hipGetLastError();
 }
 
@@ -407,16 +444,15 @@ hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
   const long long n = (long long)geo.nx * geo.ny * geo.nz;
   long long nb = (n + 255) / 256;
   if (nb > 65536) nb = 65536;
-  hipLaunchKernelGGL(adj_finish_kernel, dim3((unsigned)nb), dim3(256), 0, st,
-                     out, acc, vel, geo.sponge, geo.nx, geo.ny, geo.nz, scale);
+  hipLaunchKernelGGL(adj_finish_kernel, dim3((unsigned)nb), dim3(256), 0, st, out, acc, vel,
+                     geo.sponge, geo.nx, geo.ny, geo.nz, scale);
   return hipGetLastError();
 }
 
-hipError_t launch_transpose(float *dst, const float *src, int n0, int n1,
-                            hipStream_t st) {
+hipError_t launch_transpose(float *dst, const float *src, int n0, int n1, hipStream_t st) {
   if (n0 <= 0 || n1 <= 0) return hipSuccess;
-  hipLaunchKernelGGL(transpose_kernel, dim3((n1 + 31) / 32, (n0 + 31) / 32),
-                     dim3(32, 8), 0, st, dst, src, n0, n1);
+  hipLaunchKernelGGL(transpose_kernel, dim3((n1 + 31) / 32, (n0 + 31) / 32), dim3(32, 8), 0,
+                     st, dst, src, n0, n1);
   return hipGetLastError();
 }
 

@@ -6,16 +6,31 @@
 
 namespace born {
 
+constexpr int kMaxPts = 32;  // source points handled inside the step
+
 struct Geometry {
   int nx, ny, nz;
-  int zchunk;  // z planes per block (0: default)
   Taps taps;
   Sponge sponge;
 };
 
+// Tile shape: RY rows per thread, WY wavefronts per block (tile 64 x RY*WY).
+struct TileShape {
+  int ry, wy;
+};
+constexpr int kNumShapes = 6;
+extern const TileShape kShapes[kNumShapes];
+
+// Launch configuration of one step kernel: tile shape and z planes per block.
+struct Tune {
+  int shape;
+  int zchunk;
+};
+
+// Source points: n entries of int4 (x, y, z, 0) in device memory.
 struct SrcPts {
+  const void *p;
   int n;
-  int x[kMaxPts], y[kMaxPts], z[kMaxPts];
 };
 
 struct FwdStep {
@@ -76,18 +91,22 @@ struct LamStep {
   int id, inject;
 };
 
-hipError_t launch_fwd_step(const Geometry &geo, const FwdStep &a, hipStream_t st);
-hipError_t launch_src_step(const Geometry &geo, const SrcStep &a, hipStream_t st);
-hipError_t launch_radj_step(const Geometry &geo, const RadjStep &a, hipStream_t st);
-hipError_t launch_srec_step(const Geometry &geo, const SrecStep &a, hipStream_t st);
-hipError_t launch_lam_step(const Geometry &geo, const LamStep &a, hipStream_t st);
+hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a,
+                           hipStream_t st);
+hipError_t launch_src_step(const Geometry &geo, const Tune &tu, const SrcStep &a,
+                           hipStream_t st);
+hipError_t launch_radj_step(const Geometry &geo, const Tune &tu, const RadjStep &a,
+                            hipStream_t st);
+hipError_t launch_srec_step(const Geometry &geo, const Tune &tu, const SrecStep &a,
+                            hipStream_t st);
+hipError_t launch_lam_step(const Geometry &geo, const Tune &tu, const LamStep &a,
+                           hipStream_t st);
 hipError_t launch_fill(float *p, long long n, float v, hipStream_t st);
-hipError_t launch_src_amp(float *amp, int nsteps, int npts, const float *src,
-                          int ntsrc, const float *tab, int nsinc, int jt,
-                          int ntab, int t_off, float scale, hipStream_t st);
+hipError_t launch_src_amp(float *amp, int nsteps, int npts, const float *src, int ntsrc,
+                          const float *tab, int nsinc, int jt, int ntab, int t_off,
+                          float scale, hipStream_t st);
 hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
                              const float *vel, float scale, hipStream_t st);
-hipError_t launch_transpose(float *dst, const float *src, int n0, int n1,
-                            hipStream_t st);
+hipError_t launch_transpose(float *dst, const float *src, int n0, int n1, hipStream_t st);
 
 }  // namespace born

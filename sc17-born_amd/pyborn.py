@@ -50,6 +50,9 @@ SIGNATURES = {
     "born_set_checkpoint_budget": (_i, [_vp, C.c_size_t]),
     "born_last_checkpoint_plan": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "born_set_zchunk": (_i, [_vp, _i]),
+    "born_set_autotune": (_i, [_vp, _i]),
+    "born_get_tuning": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
+                             C.POINTER(C.c_float)]),
     "born_get_stream": (_vp, [_vp]),
     "born_synchronize": (_i, [_vp]),
 }
@@ -208,6 +211,22 @@ class Propagator:
     def set_zchunk(self, zc):
         self._chk(self.lib.born_set_zchunk(self.h, zc), "zchunk")
 
+    def set_autotune(self, on):
+        self._chk(self.lib.born_set_autotune(self.h, int(on)), "autotune")
+
+    KERNELS = ("fwd", "fwd_dV", "source", "rtm_adjoint", "background", "background_dV",
+               "adjoint_state")
+
+    def tuning(self):
+        out = {}
+        for k, name in enumerate(self.KERNELS):
+            ry, wy, zc, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+            self._chk(self.lib.born_get_tuning(self.h, k, C.byref(ry), C.byref(wy), C.byref(zc),
+                                               C.byref(ms)), "tuning")
+            out[name] = {"tile": [64, ry.value * wy.value], "rows_per_thread": ry.value,
+                         "waves": wy.value, "zchunk": zc.value, "ms": round(ms.value, 4)}
+        return out
+
     @property
     def stream(self):
         return self.lib.born_get_stream(self.h)
@@ -231,3 +250,129 @@ class Propagator:
     def dev_rtm_adjoint(self, nt, p0, p1, img, traces):
         self._chk(self.lib.born_dev_rtm_adjoint(self.h, nt, _dptr(p0), _dptr(p1), _dptr(img),
                                                 _dptr(traces)), "dev_rtm_adjoint")
+
+
+# ---------------------------------------------------------------------------
+# Born operator (include/born_host.h): rtm_zero_op on the HIP propagator
+HOST_LIB_PATH = os.path.join(_HERE, "libborn_host.so")
+_HOST = None
+
+
+class OpDesc(C.Structure):
+    """born_op_desc of include/born_host.h"""
+
+    _fields_ = [("vn", C.c_int * 3), ("vo", C.c_float * 3), ("vd", C.c_float * 3),
+                ("vel", C.c_void_p), ("nw", C.c_int), ("wdt", C.c_float),
+                ("wavelet", C.c_void_p), ("nshots", C.c_int), ("src_x", C.c_void_p),
+                ("src_y", C.c_void_p), ("src_depth", C.c_float), ("nt", C.c_int),
+                ("t0", C.c_float), ("dt", C.c_float), ("rec_nx", C.c_int), ("rec_ny", C.c_int),
+                ("rec_ox", C.c_float), ("rec_oy", C.c_float), ("rec_dx", C.c_float),
+                ("rec_dy", C.c_float), ("rec_depth", C.c_float), ("nbound", C.c_int),
+                ("mode", C.c_int), ("scatter", C.c_int), ("device", C.c_int)]
+
+
+OP_SIGNATURES = {
+    "born_op_create": (_i, [C.POINTER(_vp), C.POINTER(OpDesc)]),
+    "born_op_destroy": (_i, [_vp]),
+    "born_op_last_error": (C.c_char_p, [_vp]),
+    "born_op_sizes": (_i, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                           C.POINTER(_i), C.POINTER(_f), C.POINTER(_i)]),
+    "born_op_forward": (_i, [_vp, _i, _vp, _vp]),
+    "born_op_adjoint": (_i, [_vp, _i, _vp, _vp]),
+    "born_op_dot_test": (_i, [_vp, C.c_double, C.POINTER(C.c_double), C.POINTER(_i)]),
+    "born_op_lsrtm": (_i, [_vp, _vp, _vp, _i, C.POINTER(C.c_double)]),
+}
+
+
+def load_host_library(path=HOST_LIB_PATH):
+    global _HOST
+    if _HOST is None:
+        if not os.path.exists(path):
+            raise BornError("%s is not built" % path)
+        lib = C.CDLL(path)
+        for name, (res, args) in OP_SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _HOST = lib
+    return _HOST
+
+
+class BornOp:
+    """rtm_zero_op (base/rtm_zero_op_3d.h) on the MI355X: forward(model) ->
+    data [shot][rec_y][rec_x][t], adjoint(data) -> model on the velocity grid.
+    mode 0: the reference's RTM adjoint; mode 1: the exact transpose."""
+
+    def __init__(self, vel, vo, vd, wavelet, wdt, src_x, src_y, src_depth, nt, t0, dt,
+                 rec_n, rec_o, rec_d, rec_depth, nbound=50, mode=0, scatter=0, device=0):
+        self.lib = load_host_library()
+        vel = np.ascontiguousarray(vel, np.float32)
+        w = np.ascontiguousarray(wavelet, np.float32)
+        sx = np.ascontiguousarray(src_x, np.float32)
+        sy = np.ascontiguousarray(src_y, np.float32)
+        self._keep = (vel, w, sx, sy)
+        d = OpDesc()
+        d.vn[:] = [vel.shape[2], vel.shape[1], vel.shape[0]]
+        d.vo[:] = list(vo)
+        d.vd[:] = list(vd)
+        d.vel = vel.ctypes.data
+        d.nw, d.wdt, d.wavelet = len(w), wdt, w.ctypes.data
+        d.nshots, d.src_x, d.src_y, d.src_depth = len(sx), sx.ctypes.data, sy.ctypes.data, src_depth
+        d.nt, d.t0, d.dt = nt, t0, dt
+        d.rec_nx, d.rec_ny = rec_n
+        d.rec_ox, d.rec_oy = rec_o
+        d.rec_dx, d.rec_dy = rec_d
+        d.rec_depth, d.nbound, d.mode, d.scatter, d.device = rec_depth, nbound, mode, scatter, device
+        h = C.c_void_p()
+        rc = self.lib.born_op_create(C.byref(h), C.byref(d))
+        self.h = h
+        if rc != 0:
+            msg = self.lib.born_op_last_error(h) if h else b"create failed"
+            self.close()
+            raise BornError("born_op_create: %s" % msg.decode())
+        mn, dn, ntp, dtp = C.c_longlong(), C.c_longlong(), C.c_int(), C.c_float()
+        dom = (C.c_int * 3)()
+        self.lib.born_op_sizes(self.h, C.byref(mn), C.byref(dn), C.byref(ntp), C.byref(dtp), dom)
+        self.model_shape = vel.shape
+        self.data_shape = (len(sx), rec_n[1], rec_n[0], nt)
+        self.nt, self.dt, self.domain = ntp.value, dtp.value, tuple(dom)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.born_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise BornError("%s: %s" % (what, self.lib.born_op_last_error(self.h).decode()))
+
+    def forward(self, model, data=None, add=False):
+        m = np.ascontiguousarray(model, np.float32)
+        out = data if data is not None else np.zeros(self.data_shape, np.float32)
+        self._chk(self.lib.born_op_forward(self.h, int(add), _ptr(m), _ptr(out)), "forward")
+        return out
+
+    def adjoint(self, data, model=None, add=False):
+        d = np.ascontiguousarray(data, np.float32)
+        out = model if model is not None else np.zeros(self.model_shape, np.float32)
+        self._chk(self.lib.born_op_adjoint(self.h, int(add), _ptr(out), _ptr(d)), "adjoint")
+        return out
+
+    def dot_test(self, tol=1e-4):
+        dots = (C.c_double * 4)()
+        ok = C.c_int()
+        self._chk(self.lib.born_op_dot_test(self.h, tol, dots, C.byref(ok)), "dot_test")
+        return bool(ok.value), list(dots)
+
+    def lsrtm(self, data, niter):
+        d = np.ascontiguousarray(data, np.float32)
+        m = np.zeros(self.model_shape, np.float32)
+        res = (C.c_double * max(niter, 1))()
+        self._chk(self.lib.born_op_lsrtm(self.h, _ptr(d), _ptr(m), niter, res), "lsrtm")
+        return m, list(res)[:niter]

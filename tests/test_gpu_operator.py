@@ -1,0 +1,77 @@
+"""The Born operator (rtm_zero_op forward/adjoint behind my_operator) on the
+MI355X against the oracle composition of the reference's operator
+(base/rtm_zero_op_3d.cc:101-299), plus its dot test and CG."""
+import numpy as np
+import pytest
+
+from cases import rel_l2
+from oracle.op import OracleBornOp
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(seed=0, nshots=2):
+    rng = np.random.default_rng(seed)
+    vn = (40, 36, 30)  # x, y, z model samples
+    d = 10.0
+    z = np.arange(vn[2], dtype=np.float32) * np.float32(d)
+    vz = np.clip(1500 + 15 * np.sqrt(z), 1500, 3000).astype(np.float32)
+    vel = np.ascontiguousarray(np.broadcast_to(vz[:, None, None], (vn[2], vn[1], vn[0])),
+                               np.float32)
+    nw, wdt = 80, 0.001
+    t = np.arange(nw) * wdt - 0.04
+    a = (np.pi * 25 * t) ** 2
+    wav = ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
+    sx = rng.uniform(100, 290, nshots).astype(np.float32)
+    sy = rng.uniform(100, 250, nshots).astype(np.float32)
+    kw = dict(vo=(0.0, 0.0, 0.0), vd=(d, d, d), wavelet=wav, wdt=wdt, src_x=sx, src_y=sy,
+              src_depth=20.0, nt=100, t0=0.0, dt=0.001, rec_n=(18, 16), rec_o=(10.0, 10.0),
+              rec_d=(20.0, 20.0), rec_depth=10.0, nbound=8)
+    model = rng.standard_normal((vn[2], vn[1], vn[0])).astype(np.float32)
+    model[:5] = 0
+    return vel, kw, model
+
+
+def test_operator_forward_and_reference_adjoint_match_oracle(pyborn):
+    vel, kw, model = _setup()
+    O = OracleBornOp(vel.shape[::-1], kw["vo"], kw["vd"], vel, kw["wavelet"], kw["wdt"],
+                     kw["src_x"], kw["src_y"], kw["src_depth"], kw["nt"], kw["t0"], kw["dt"],
+                     kw["rec_n"], kw["rec_o"], kw["rec_d"], kw["rec_depth"], kw["nbound"])
+    H = pyborn.BornOp(vel, mode=0, **kw)
+    assert H.nt == O.nt and H.domain == tuple(O.dn)
+    d_o = O.forward(model)
+    d_h = H.forward(model)
+    assert np.abs(d_o).max() > 0
+    assert rel_l2(d_h, d_o) <= 1e-4
+    rng = np.random.default_rng(1)
+    data = rng.standard_normal(H.data_shape).astype(np.float32)
+    m_o = O.adjoint(data)
+    m_h = H.adjoint(data)
+    assert np.abs(m_o).max() > 0
+    assert rel_l2(m_h, m_o) <= 1e-4
+    # add=True accumulates (my_operator.h:18-25 semantics)
+    m2 = H.adjoint(data, model=m_h.copy(), add=True)
+    assert rel_l2(m2, 2 * m_h) <= 1e-6
+    H.close()
+
+
+@pytest.mark.parametrize("scatter", [0, 1])
+def test_exact_operator_passes_my_operator_dot_test(pyborn, scatter):
+    vel, kw, _ = _setup(seed=3)
+    H = pyborn.BornOp(vel, mode=1, scatter=scatter, **kw)
+    ok, dots = H.dot_test(tol=1e-5)
+    H.close()
+    assert ok, dots
+    assert abs(dots[0] - dots[1]) <= 1e-5 * abs(dots[0])
+
+
+def test_lsrtm_reduces_the_residual(pyborn):
+    vel, kw, model = _setup(seed=5, nshots=1)
+    H = pyborn.BornOp(vel, mode=1, **kw)
+    data = H.forward(model)
+    m, res = H.lsrtm(data, 4)
+    H.close()
+    r0 = float(np.dot(data.ravel().astype(np.float64), data.ravel()))
+    assert all(r > 0 for r in res)
+    assert res[0] < r0 and all(b <= a * (1 + 1e-6) for a, b in zip(res, res[1:]))
+    assert res[-1] < 0.5 * r0
