@@ -95,6 +95,11 @@ def main():
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--budget-gb", type=float, default=0.0,
                     help="HBM budget for the background history (0: free memory)")
+    ap.add_argument("--nt", type=int, default=0, help="time steps (profiling runs)")
+    ap.add_argument("--shots", type=int, default=0, help="shots (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device to device (a kernel of known traffic "
+                         "for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one shot batch per GPU and "
@@ -106,12 +111,24 @@ def main():
     pyborn = load("pyborn")
     configs = load("configs")
     cfg = dict(configs.CONFIGS[args.config])
+    if args.nt:
+        cfg["nt"] = args.nt
+    if args.shots:
+        cfg["nshots"] = args.shots
     t_setup = time.perf_counter()
     p = configs.make_problem(**cfg)
     log("problem built in %.1f s: %dx%dx%d, %d shots, nt=%d"
         % (time.perf_counter() - t_setup, p.nx, p.ny, p.nz, len(p.shots), p.nt))
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        a = torch.zeros(256 * 2 ** 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("copy calibration: 3 x (1 GiB read + 1 GiB write)")
     P = pyborn.Propagator(0)
     P.create_space(p.d, p.d, p.d, p.nx, p.ny, p.nz)
     P.transfer_sinc_table_s(p.tab)
