@@ -29,6 +29,11 @@ HBM_PEAK_GBS = 8000.0
 FWD_BYTES_PER_CELL = 32      # s and r: read cur, read old, write new; vel; model
 ADJ_BYTES_PER_CELL = 36      # background recompute 12, adjoint state 12, vel 4, image r/w 8
 METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs FP32 plain"
+# HBM bytes per launch of the forward step kernel on the 304^3 workload, from
+# rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of this script
+# (profiles/born_fwd_adj_304cube_hbm_counters.md); reported as roofline.traffic
+# only when the measured workload and kernel are the profiled ones.
+PROFILED_TRAFFIC = {("c2", "FwdOp<0>, 2, 4", "f32"): 1301.5e6}
 
 
 def load(name):
@@ -44,7 +49,7 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p, nthreads, target_s=12.0):
+def cpu_baseline(p, nthreads, target_s=15.0):
     """The oracle (C restatement of the reference's cpu_prop.cc) on this host:
     rtmForward + sourceProp + rtmAdjoint on the same domain, a bounded number
     of steps, timed with all threads given."""
@@ -72,7 +77,9 @@ def cpu_baseline(p, nthreads, target_s=12.0):
         return time.perf_counter() - t0
 
     t4 = run(4)
-    nt = int(max(4, min(400, 4 * target_s / max(t4, 1e-3))))
+    t12 = run(12)
+    per_step = max((t12 - t4) / 8, 1e-4)
+    nt = int(max(4, min(1500, target_s / per_step)))
     t = run(nt)
     cell_steps = p.interior_cells * nt
     return {
@@ -193,6 +200,11 @@ def main():
     fwd_alg_bytes = FWD_BYTES_PER_CELL * p.interior_cells
     achieved = fwd_alg_bytes / fwd_launch_s / 1e9
     alg_gbs = (FWD_BYTES_PER_CELL + ADJ_BYTES_PER_CELL) * cell_steps / elapsed / 1e9
+    ft = tuning["fwd"]
+    fwd_kernel = "FwdOp<0>, %d, %d" % (ft["rows_per_thread"], ft["waves"])
+    traffic = PROFILED_TRAFFIC.get((args.config, fwd_kernel, "f32"))
+    if args.nt or args.shots:
+        traffic = None
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -223,12 +235,12 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "march_kernel<FwdOp<0>> (fused Born forward step)",
+            "kernel": "march_kernel<%s> (fused Born forward step)" % fwd_kernel,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_cell": FWD_BYTES_PER_CELL,
             "avg_launch_us": round(fwd_launch_s * 1e6, 2),
         },

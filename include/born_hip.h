@@ -68,7 +68,7 @@ int born_transfer_sinc_table_d(born_ctx *ctx, int nsinc, int jtd,
                                const float *table);
 
 /* transfer_source_func (gpu_funcs_3d.h:23) / transferSourceFunc
- * (base_prop.h:22).  vals is [npts][nt_big]; at most 16 points, all inside the
+ * (base_prop.h:22).  vals is [npts][nt_big]; at most 32 points, all inside the
  * stencil interior. */
 int born_transfer_source_func(born_ctx *ctx, int npts, int nt_big,
                               const int *locs, const float *vals);
