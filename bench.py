@@ -28,6 +28,7 @@ PKG = os.path.join(ROOT, "sc17-born_amd")
 HBM_PEAK_GBS = 8000.0
 FWD_BYTES_PER_CELL = 32      # s and r: read cur, read old, write new; vel; model
 ADJ_BYTES_PER_CELL = 36      # background recompute 12, adjoint state 12, vel 4, image r/w 8
+F16_BYTES_PER_CELL = 44      # the same with 2-byte wavefield levels (vel, model, image FP32)
 METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs FP32 plain"
 # HBM bytes per launch of the forward step kernel on the 304^3 workload, from
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of this script
@@ -104,6 +105,7 @@ def main():
                     help="HBM budget for the background history (0: free memory)")
     ap.add_argument("--nt", type=int, default=0, help="time steps (profiling runs)")
     ap.add_argument("--shots", type=int, default=0, help="shots (profiling runs)")
+    ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (a kernel of known traffic "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -154,41 +156,52 @@ def main():
     torch.cuda.synchronize()
     stream = torch.cuda.ExternalStream(P.stream)
     nt = p.nt
-    fwd_events = []
-    adj_events = []
 
-    def step(record):
-        img.zero_()
-        torch.cuda.synchronize()
-        for locs, vals in p.shots:
-            P.transfer_source_func(locs, vals)
-            traces.zero_()
+    def run_leg(storage):
+        """warmup + timed steps with the given wavefield storage; returns
+        (elapsed s, forward events, adjoint events)"""
+        P.set_storage(storage)
+        fwd_events, adj_events = [], []
+
+        def step(record):
+            img.zero_()
             torch.cuda.synchronize()
-            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-            e[0].record(stream)
-            P.dev_forward(1, nt, model, traces, scatter=0, keep_background=True)
-            e[1].record(stream)
-            P.dev_adjoint_exact(1, nt, traces, img, scatter=0)
-            e[2].record(stream)
-            if record:
-                fwd_events.append((e[0], e[1]))
-                adj_events.append((e[1], e[2]))
-        P.synchronize()
+            for locs, vals in p.shots:
+                P.transfer_source_func(locs, vals)
+                traces.zero_()
+                torch.cuda.synchronize()
+                e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+                e[0].record(stream)
+                P.dev_forward(1, nt, model, traces, scatter=0, keep_background=True)
+                e[1].record(stream)
+                P.dev_adjoint_exact(1, nt, traces, img, scatter=0)
+                e[2].record(stream)
+                if record:
+                    fwd_events.append((e[0], e[1]))
+                    adj_events.append((e[1], e[2]))
+            P.synchronize()
 
-    for i in range(args.warmup):
-        step(False)
-        log("warmup step %d done" % i)
-    torch.cuda.synchronize()
-    P.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(True)
-        log("timed step %d done" % i)
-    torch.cuda.synchronize()
-    P.synchronize()
-    elapsed = time.perf_counter() - t0
+        for i in range(args.warmup):
+            step(False)
+            log("%s warmup step %d done" % (storage, i))
+        torch.cuda.synchronize()
+        P.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step(True)
+            log("%s timed step %d done" % (storage, i))
+        torch.cuda.synchronize()
+        P.synchronize()
+        return time.perf_counter() - t0, fwd_events, adj_events
+
+    elapsed, fwd_events, adj_events = run_leg("f32")
     segs, recomputed = P.last_checkpoint_plan()
     tuning = P.tuning()
+    f16 = None
+    if not args.no_f16:
+        el16, fe16, ae16 = run_leg("f16")
+        f16 = (el16, fe16, ae16, P.last_checkpoint_plan())
+        tuning = P.tuning()
 
     nshots = len(p.shots)
     cell_steps = p.interior_cells * nt * nshots * args.steps
@@ -245,6 +258,20 @@ def main():
             "avg_launch_us": round(fwd_launch_s * 1e6, 2),
         },
     }
+    if f16 is not None:
+        el16, fe16, ae16, (segs16, rec16) = f16
+        v16 = 4 * cell_steps / el16 / 1e6
+        out["config"]["f16_storage"] = {
+            "value": round(v16, 1),
+            "unit": "Mcell-updates/s",
+            "ms_per_step": round(el16 / args.steps * 1e3, 2),
+            "speedup_vs_f32": round(v16 / value, 3),
+            "forward_ms_per_shot": round(float(np.mean([a.elapsed_time(b) for a, b in fe16])), 2),
+            "adjoint_ms_per_shot": round(float(np.mean([a.elapsed_time(b) for a, b in ae16])), 2),
+            "checkpoint_segments": segs16,
+            "recomputed_steps": rec16,
+            "bytes_per_born_cell_step": F16_BYTES_PER_CELL,
+        }
     P.close()
     if not args.no_cpu_baseline:
         log("cpu baseline")

@@ -146,6 +146,18 @@ int born_set_checkpoint_budget(born_ctx *ctx, size_t bytes);
 /* segments and recomputed steps of the last exact adjoint */
 int born_last_checkpoint_plan(const born_ctx *ctx, int *segments,
                               int *recomputed_steps);
+/* Storage of the wavefield levels and background history of the Born
+ * forward and exact adjoint (born_dev_forward, born_dev_adjoint_exact and
+ * their host-pointer forms).  BORN_STORAGE_F16 keeps every level as FP16
+ * with a power-of-two scale per level, chosen on the device from a bound on
+ * the level's magnitude; arithmetic stays FP32, and traces, model and image
+ * stay FP32.  No reference counterpart: the reference stores FP32 only.
+ * The reference-mode loops (born_source_prop, born_rtm_adjoint) always use
+ * FP32. */
+#define BORN_STORAGE_F32 0
+#define BORN_STORAGE_F16 1
+int born_set_storage(born_ctx *ctx, int storage);
+
 /* z planes per block of the step kernels (0 = chosen by the autotuner) */
 int born_set_zchunk(born_ctx *ctx, int zchunk);
 
@@ -156,7 +168,8 @@ int born_set_zchunk(born_ctx *ctx, int zchunk);
 int born_set_autotune(born_ctx *ctx, int on);
 /* The chosen launch shape of a step kernel: 0 Born forward, 1 Born forward
  * (dV scatter), 2 source propagation, 3 reference adjoint, 4/5 background
- * recompute, 6 adjoint state; ms = timed duration (-1 when not tuned). */
+ * recompute, 6 adjoint state; 7-11 the FP16-storage forms of 0, 1, 4, 5, 6.
+ * ms = timed duration (-1 when not tuned). */
 int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
                     int *waves_per_block, int *zchunk, float *ms);
 

@@ -42,8 +42,8 @@ struct born_ctx {
   float *d_bx = nullptr, *d_by = nullptr, *d_bz = nullptr;
   int zchunk = 0;  // override of the tuned z chunk (0: tuned)
   int autotune = 1;
-  born::Tune tune[7];
-  float tune_ms[7];
+  born::Tune tune[12];
+  float tune_ms[12];
 
   int nsinc_s = 0, jts = 0, nsinc_d = 0, jtd = 0;
   float *d_tab_s = nullptr, *d_tab_d = nullptr;
@@ -61,6 +61,17 @@ struct born_ctx {
 
   int jt_state = 1;
 
+  // wavefield storage of the Born forward / exact adjoint (BORN_STORAGE_*)
+  int storage = BORN_STORAGE_F32;
+  float vmax2 = 0.f;      // max of velocity 2 (V = v^2 dt^2)
+  float tabd_abs = 0.f;   // largest sum |tab_d[ii][k]| over the rows
+  int rec_per_cell = 0;   // most receivers sharing one cell
+  float *d_ring = nullptr;    // FP16 level scales, see ring()
+  float *d_hscale = nullptr;  // FP16 scale per history level
+  size_t hscale_cap = 0;
+  float *d_ckring = nullptr;  // ring snapshot per checkpoint (8 words each)
+  size_t ckring_cap = 0;
+
   float *d_w[4] = {nullptr, nullptr, nullptr, nullptr};
   float *d_img = nullptr, *d_acc = nullptr, *d_p0 = nullptr, *d_p1 = nullptr;
   float *d_tr = nullptr, *d_tr2 = nullptr;
@@ -75,7 +86,7 @@ struct born_ctx {
   size_t ckpt_cap = 0;  // fields
   long long bg_version = 1;
   long long hist_version = -1;
-  int hist_scatter = -1, hist_jt = -1, hist_nt = -1;
+  int hist_scatter = -1, hist_jt = -1, hist_nt = -1, hist_storage = -1;
   int last_segments = 0, last_recomputed = 0;
 };
 
@@ -164,7 +175,10 @@ born::RecMap recmap(const born_ctx *c) {
 }
 
 // step kernels that are tuned separately
-enum Kind { K_FWD0, K_FWD1, K_SRC, K_RADJ, K_SREC0, K_SREC1, K_LAM, K_NUM };
+enum Kind {
+  K_FWD0, K_FWD1, K_SRC, K_RADJ, K_SREC0, K_SREC1, K_LAM,
+  K_FWD0H, K_FWD1H, K_SREC0H, K_SREC1H, K_LAMH, K_NUM
+};
 
 born::SrcPts points(const born_ctx *c) { return {c->d_pts, c->npts_s}; }
 born::SrcPts no_points() { return {nullptr, 0}; }
@@ -314,27 +328,28 @@ size_t free_bytes(born_ctx *ctx) {
 // Bytes available for background history: the set budget, or the free
 // memory plus what the history buffers already hold, less a reserve.
 size_t history_budget(born_ctx *ctx) {
-  const size_t F = (size_t)ctx->n * sizeof(float);
   if (ctx->budget) return ctx->budget;
-  const size_t held = (ctx->seg_cap + ctx->ckpt_cap) * F;
+  const size_t held = (ctx->seg_cap + ctx->ckpt_cap) * sizeof(float);
   const size_t reserve = (size_t)2 << 30;
   const size_t fr = free_bytes(ctx) + held;
   return fr > reserve ? fr - reserve : 0;
 }
 
-// history buffers: L fields of scatter operand + 2*(S-1) checkpoint fields
-int history_alloc(born_ctx *ctx, size_t L, size_t nck) {
-  const size_t F = (size_t)ctx->n;
-  if (ctx->seg_cap < L) {
+// history buffers: L levels of scatter operand + nck checkpoint levels, each
+// n elements of esz bytes (capacities are counted in floats)
+int history_alloc(born_ctx *ctx, size_t L, size_t nck, size_t esz) {
+  const size_t seg = (L * (size_t)ctx->n * esz + 3) / 4;
+  const size_t ck = (nck * (size_t)ctx->n * esz + 3) / 4;
+  if (ctx->seg_cap < seg) {
     ctx->seg_cap = 0;
     ctx->hist_version = -1;
-    CHK(dev_alloc(ctx, &ctx->d_seg, L * F));
-    ctx->seg_cap = L;
+    CHK(dev_alloc(ctx, &ctx->d_seg, seg));
+    ctx->seg_cap = seg;
   }
-  if (ctx->ckpt_cap < nck) {
+  if (ctx->ckpt_cap < ck) {
     ctx->ckpt_cap = 0;
-    CHK(dev_alloc(ctx, &ctx->d_ckpt, nck * F));
-    ctx->ckpt_cap = nck;
+    CHK(dev_alloc(ctx, &ctx->d_ckpt, ck));
+    ctx->ckpt_cap = ck;
   }
   return BORN_OK;
 }
@@ -367,6 +382,55 @@ int traces_out(born_ctx *ctx, float *h) {
   HIPCHK(hipMemcpyAsync(h, ctx->d_tr2, bytes, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return BORN_OK;
+}
+
+size_t elem(const born_ctx *c) { return c->storage == BORN_STORAGE_F16 ? 2 : 4; }
+
+// k-th field of element size esz in an array of fields of n cells
+float *nth(float *base, long long k, long long n, size_t esz) {
+  return (float *)((char *)base + (size_t)k * n * esz);
+}
+
+// ctx->d_ring (64 words): rings s, r, u at 0, 8, 16 (scale[4], maxb[4]);
+// max |model| at 24, max |trace| at 25; scratch rings for the autotuner at
+// 32, 40, with scratch max at 48 and scratch history scale at 49.
+enum { RING_S = 0, RING_R = 8, RING_U = 16, MAX_M = 24, MAX_TR = 25, RING_T0 = 32,
+       RING_T1 = 40, MAX_T = 48, HS_T = 49 };
+born::Ring ring(born_ctx *c, int at) {
+  return {c->d_ring + at, (unsigned *)(c->d_ring + at + 4)};
+}
+const unsigned *maxp(born_ctx *c, int at) { return (const unsigned *)(c->d_ring + at); }
+
+int ring_buffers(born_ctx *ctx) {
+  if (!ctx->d_ring) {
+    CHK(dev_alloc(ctx, &ctx->d_ring, 64));
+    HIPCHK(born::launch_fill(ctx->d_ring, 64, 0.f, ctx->stream));
+  }
+  return BORN_OK;
+}
+
+int ring_reset(born_ctx *ctx, int at) {
+  born::Ring r = ring(ctx, at);
+  HIPCHK(born::launch_ring_init(r.scale, r.maxb, 4, ctx->stream));
+  return BORN_OK;
+}
+
+// slots of forward step k (reads levels k, k-1; writes k+1)
+born::Slots slots(int k) { return {k % 4, (k + 3) % 4, (k + 1) % 4, (k + 2) % 4}; }
+
+float tap_sum(const born_ctx *c) {
+  float t = fabsf(c->coeffs[0]);
+  for (int i = 1; i < 13; i++) t += 2.f * fabsf(c->coeffs[i]);
+  return t;
+}
+
+born::Half half_state(const born_ctx *c, int k) {
+  born::Half h;
+  h.on = c->storage == BORN_STORAGE_F16;
+  h.sl = slots(k);
+  h.tapsum = tap_sum(c);
+  h.gain = 2.f + c->vmax2 * h.tapsum;
+  return h;
 }
 
 int field_buf(born_ctx *ctx, float **p) {
@@ -524,8 +588,16 @@ int born_transfer_sinc_table_s(born_ctx *ctx, int nsinc, int jts, const float *t
 }
 
 int born_transfer_sinc_table_d(born_ctx *ctx, int nsinc, int jtd, const float *table) {
-  return sinc_table(ctx, nsinc, jtd, table, ctx ? &ctx->d_tab_d : nullptr,
-                    ctx ? &ctx->nsinc_d : nullptr, ctx ? &ctx->jtd : nullptr);
+  CHK(sinc_table(ctx, nsinc, jtd, table, ctx ? &ctx->d_tab_d : nullptr,
+                 ctx ? &ctx->nsinc_d : nullptr, ctx ? &ctx->jtd : nullptr));
+  float m = 0.f;
+  for (int r = 0; r < jtd; r++) {
+    float a = 0.f;
+    for (int k = 0; k < nsinc; k++) a += fabsf(table[(size_t)r * nsinc + k]);
+    m = std::max(m, a);
+  }
+  ctx->tabd_abs = m;
+  return BORN_OK;
 }
 
 int born_transfer_source_func(born_ctx *ctx, int npts, int nt_big, const int *locs,
@@ -572,6 +644,11 @@ int born_transfer_vel_func1(born_ctx *ctx, int nx, int ny, int nz, const float *
 }
 
 int born_transfer_vel_func2(born_ctx *ctx, int nx, int ny, int nz, const float *v) {
+  if (ctx && v && nx == ctx->nx && ny == ctx->ny && nz == ctx->nz) {
+    float m = 0.f;
+    for (long long i = 0; i < ctx->n; i++) m = std::max(m, fabsf(v[i]));
+    ctx->vmax2 = m;
+  }
   return vel_func(ctx, nx, ny, nz, v, ctx ? &ctx->d_vel2 : nullptr);
 }
 
@@ -592,6 +669,8 @@ int born_transfer_receiver_func(born_ctx *ctx, int rec_nx, int rec_ny, int nt_bi
   }
   const long long n12 = (long long)ctx->nx * ctx->ny;
   std::vector<int> rmap((size_t)nslots * n12, -1), rnext(nrec, -1), last((size_t)nslots * n12, -1);
+  std::vector<int> count((size_t)nslots * n12, 0);
+  int per_cell = 0;
   for (int i = 0; i < nrec; i++) {
     const long long c = slot_of_z[pz[i]] * n12 + (long long)py[i] * ctx->nx + px[i];
     if (rmap[c] < 0)
@@ -599,6 +678,7 @@ int born_transfer_receiver_func(born_ctx *ctx, int rec_nx, int rec_ny, int nt_bi
     else
       rnext[last[c]] = i;
     last[c] = i;
+    per_cell = std::max(per_cell, ++count[c]);
   }
   CHK(set_device(ctx));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -616,6 +696,7 @@ int born_transfer_receiver_func(born_ctx *ctx, int rec_nx, int rec_ny, int nt_bi
   ctx->nrec = nrec;
   ctx->nt_rec = nt_big;
   ctx->nslots = nslots;
+  ctx->rec_per_cell = per_cell;
   ctx->h_rec = traces;
   return BORN_OK;
 }
@@ -630,29 +711,45 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     return fail(ctx, BORN_ERR_ARG, "bad forward arguments");
   CHK(set_device(ctx));
   CHK(ensure_work(ctx));
+  CHK(ring_buffers(ctx));
   const long long n = ctx->n;
+  const bool half = ctx->storage == BORN_STORAGE_F16;
+  const size_t esz = elem(ctx);
   float *hist = nullptr;
   if (keep_background && nt > 0) {
-    if ((size_t)nt * n * sizeof(float) <= history_budget(ctx)) {
-      CHK(history_alloc(ctx, (size_t)nt, 0));
+    if ((size_t)nt * n * esz <= history_budget(ctx)) {
+      CHK(history_alloc(ctx, (size_t)nt, 0, esz));
       hist = ctx->d_seg;
+      if (half) CHK(ensure(ctx, &ctx->d_hscale, &ctx->hscale_cap, (size_t)nt));
     }
   }
   ctx->hist_version = -1;
   ctx->jt_state = jt;
   const Geometry geo = geometry(ctx);
-  const int kind = scatter == 0 ? K_FWD0 : K_FWD1;
-  {  // tune on the work fields (zeroed below), no traces written
+  const int kind = half ? (scatter == 0 ? K_FWD0H : K_FWD1H) : (scatter == 0 ? K_FWD0 : K_FWD1);
+  {  // tune on the work fields (zeroed below) and scratch scales, no traces written
+    CHK(ring_reset(ctx, RING_T0));
+    CHK(ring_reset(ctx, RING_T1));
+    HIPCHK(born::launch_fill(ctx->d_ring + MAX_T, 2, 0.f, ctx->stream));
     born::FwdStep t{};
     t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.r1 = ctx->d_w[3]; t.r0 = ctx->d_w[2];
     t.vel = ctx->d_vel2; t.model = model; t.src = no_points(); t.rec = recmap(ctx);
     t.tr = traces; t.tab = ctx->d_tab_d; t.extract = 0; t.scatter = scatter;
+    t.h = half_state(ctx, 0);
+    t.ring[0] = ring(ctx, RING_T0); t.ring[1] = ring(ctx, RING_T1);
+    t.mmax = maxp(ctx, MAX_T); t.hscale = ctx->d_ring + HS_T;
     CHK(autotune(ctx, kind, [&](const born::Tune &tu) {
       return born::launch_fwd_step(geo, tu, t, ctx->stream); }));
   }
   const born::Tune tu = tuning(ctx, kind);
   for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w[i], n));
   CHK(source_amps(ctx, nt, jt, 0, 1 / (float)jt));
+  if (half) {
+    CHK(ring_reset(ctx, RING_S));
+    CHK(ring_reset(ctx, RING_R));
+    HIPCHK(born::launch_fill(ctx->d_ring + MAX_M, 1, 0.f, ctx->stream));
+    HIPCHK(born::launch_absmax(model, n, (unsigned *)(ctx->d_ring + MAX_M), ctx->stream));
+  }
   float *s_old = ctx->d_w[0], *s_cur = ctx->d_w[1];
   float *r_old = ctx->d_w[2], *r_cur = ctx->d_w[3];
   born::FwdStep a{};
@@ -662,17 +759,22 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   a.rec = recmap(ctx);
   a.tr = traces;
   a.scatter = scatter;
+  a.ring[0] = ring(ctx, RING_S);
+  a.ring[1] = ring(ctx, RING_R);
+  a.mmax = maxp(ctx, MAX_M);
   for (int it = 0; it < nt; it++) {
     const int id = it / jt, ii = it - id * jt;
     a.s1 = s_cur;
     a.s0 = s_old;
     a.r1 = r_cur;
     a.r0 = r_old;
-    a.hist = hist ? hist + (long long)it * n : nullptr;
+    a.hist = hist ? nth(hist, it, n, esz) : nullptr;
+    a.hscale = half && hist ? ctx->d_hscale + it : nullptr;
     a.amp = ctx->d_amp + (long long)it * ctx->npts_s;
     a.tab = ctx->d_tab_d + (long long)ii * 8;
     a.id = id;
     a.extract = id + 7 < ctx->nt_rec;
+    a.h = half_state(ctx, it);
     HIPCHK(born::launch_fwd_step(geo, tu, a, ctx->stream));
     std::swap(s_old, s_cur);
     std::swap(r_old, r_cur);
@@ -682,6 +784,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     ctx->hist_scatter = scatter;
     ctx->hist_jt = jt;
     ctx->hist_nt = nt;
+    ctx->hist_storage = ctx->storage;
   }
   return BORN_OK;
 }
@@ -782,32 +885,48 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
     return fail(ctx, BORN_ERR_ARG, "bad adjoint arguments");
   CHK(set_device(ctx));
   CHK(ensure_work(ctx));
+  CHK(ring_buffers(ctx));
   CHK(field_buf(ctx, &ctx->d_acc));
   const long long n = ctx->n;
   if (nt == 0) return BORN_OK;
+  const bool half = ctx->storage == BORN_STORAGE_F16;
+  const size_t esz = elem(ctx);
   const Geometry geo = geometry(ctx);
-  const int skind = scatter == 0 ? K_SREC0 : K_SREC1;
-  {  // tune both kernels on work fields and the accumulator (zeroed below)
+  const int skind = half ? (scatter == 0 ? K_SREC0H : K_SREC1H)
+                         : (scatter == 0 ? K_SREC0 : K_SREC1);
+  const int lkind = half ? K_LAMH : K_LAM;
+  // bound of |V B P^T w| per unit of max |trace| (FP16 scales)
+  const float inj = ctx->vmax2 * ctx->tabd_abs * (float)ctx->rec_per_cell;
+  {  // tune both kernels on work fields, the accumulator and scratch scales
+    CHK(ring_reset(ctx, RING_T0));
+    HIPCHK(born::launch_fill(ctx->d_ring + MAX_T, 1, 0.f, ctx->stream));
+    HIPCHK(born::launch_fill(ctx->d_ring + HS_T, 1, 1.f, ctx->stream));
     born::SrecStep t{};
     t.s1 = ctx->d_w[3]; t.s0 = ctx->d_w[2]; t.vel = ctx->d_vel2; t.sig = ctx->d_acc;
     t.src = no_points(); t.scatter = scatter;
+    t.h = half_state(ctx, 0); t.ring = ring(ctx, RING_T0); t.hscale = ctx->d_ring + HS_T;
     CHK(autotune(ctx, skind, [&](const born::Tune &tu) {
       return born::launch_srec_step(geo, tu, t, ctx->stream); }));
+    CHK(ring_reset(ctx, RING_T0));
+    HIPCHK(born::launch_fill(ctx->d_ring + HS_T, 1, 1.f, ctx->stream));
     born::LamStep l{};
     l.u1 = ctx->d_w[1]; l.u0 = ctx->d_w[0]; l.vel = ctx->d_vel2; l.sig = ctx->d_w[2];
     l.acc = ctx->d_acc; l.rec = recmap(ctx); l.tr = traces; l.tab = ctx->d_tab_d;
     l.inject = 0;
-    CHK(autotune(ctx, K_LAM, [&](const born::Tune &tu) {
+    l.h = half_state(ctx, 0); l.ring = ring(ctx, RING_T0); l.inj = inj;
+    l.trmax = maxp(ctx, MAX_T); l.hscale = ctx->d_ring + HS_T;
+    CHK(autotune(ctx, lkind, [&](const born::Tune &tu) {
       return born::launch_lam_step(geo, tu, l, ctx->stream); }));
   }
-  const born::Tune stu = tuning(ctx, skind), ltu = tuning(ctx, K_LAM);
+  const born::Tune stu = tuning(ctx, skind), ltu = tuning(ctx, lkind);
   const bool cached = ctx->hist_version == ctx->bg_version &&
                       ctx->hist_scatter == scatter && ctx->hist_jt == jt &&
-                      ctx->hist_nt == nt && ctx->seg_cap >= (size_t)nt;
+                      ctx->hist_nt == nt && ctx->hist_storage == ctx->storage &&
+                      ctx->seg_cap * sizeof(float) >= (size_t)nt * n * esz;
   int S = 1, L = nt;
   if (!cached) {
     ctx->hist_version = -1;  // the segment buffer is about to be reused
-    const size_t F = (size_t)n * sizeof(float);
+    const size_t F = (size_t)n * esz;
     const size_t avail = history_budget(ctx);
     S = 0;
     for (int s = 1; s <= nt; s++) {
@@ -822,7 +941,11 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
       return fail(ctx, BORN_ERR_NOMEM, "no checkpoint plan fits the HBM budget");
     L = (nt + S - 1) / S;
     S = (nt + L - 1) / L;
-    CHK(history_alloc(ctx, (size_t)L, 2 * (size_t)(S - 1)));
+    CHK(history_alloc(ctx, (size_t)L, 2 * (size_t)(S - 1), esz));
+    if (half) {
+      CHK(ensure(ctx, &ctx->d_hscale, &ctx->hscale_cap, (size_t)L));
+      CHK(ensure(ctx, &ctx->d_ckring, &ctx->ckring_cap, 8 * (size_t)std::max(S - 1, 1)));
+    }
   }
   CHK(source_amps(ctx, nt, jt, 0, 1 / (float)jt));
   float *s_old = ctx->d_w[2], *s_cur = ctx->d_w[3];
@@ -830,24 +953,33 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   sr.vel = ctx->d_vel2;
   sr.src = points(ctx);
   sr.scatter = scatter;
-  const size_t Fb = (size_t)n * sizeof(float);
+  sr.ring = ring(ctx, RING_S);
+  const size_t Fb = (size_t)n * esz;
   int recomputed = 0;
   if (!cached) {
-    // pass 1: background sweep; checkpoints at segment starts, the last
-    // segment's scatter operands stored directly
+    // pass 1: background sweep; checkpoints at segment starts (with the
+    // FP16 scale ring), the last segment's scatter operands stored directly
     CHK(zero(ctx, s_old, n));
     CHK(zero(ctx, s_cur, n));
+    if (half) CHK(ring_reset(ctx, RING_S));
     const int klast = (S - 1) * L;
     for (int k = 0; k < nt; k++) {
       if (k > 0 && k % L == 0 && k / L <= S - 1) {
-        float *ck = ctx->d_ckpt + (long long)2 * (k / L - 1) * n;
+        const long long c = k / L - 1;
+        float *ck = nth(ctx->d_ckpt, 2 * c, n, esz);
         HIPCHK(hipMemcpyAsync(ck, s_cur, Fb, hipMemcpyDeviceToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ck + n, s_old, Fb, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(nth(ctx->d_ckpt, 2 * c + 1, n, esz), s_old, Fb,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        if (half)
+          HIPCHK(hipMemcpyAsync(ctx->d_ckring + 8 * c, ctx->d_ring + RING_S, 8 * sizeof(float),
+                                hipMemcpyDeviceToDevice, ctx->stream));
       }
       sr.s1 = s_cur;
       sr.s0 = s_old;
-      sr.sig = k >= klast ? ctx->d_seg + (long long)(k - klast) * n : nullptr;
+      sr.sig = k >= klast ? nth(ctx->d_seg, k - klast, n, esz) : nullptr;
+      sr.hscale = half && sr.sig ? ctx->d_hscale + (k - klast) : nullptr;
       sr.amp = ctx->d_amp + (long long)k * ctx->npts_s;
+      sr.h = half_state(ctx, k);
       HIPCHK(born::launch_srec_step(geo, stu, sr, ctx->stream));
       std::swap(s_old, s_cur);
     }
@@ -857,41 +989,60 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   CHK(zero(ctx, u_old, n));
   CHK(zero(ctx, u_cur, n));
   CHK(zero(ctx, ctx->d_acc, n));
+  if (half) {
+    CHK(ring_reset(ctx, RING_U));
+    HIPCHK(born::launch_fill(ctx->d_ring + MAX_TR, 1, 0.f, ctx->stream));
+    HIPCHK(born::launch_absmax(traces, (long long)ctx->nrec * ctx->nt_rec,
+                               (unsigned *)(ctx->d_ring + MAX_TR), ctx->stream));
+  }
   born::LamStep lm{};
   lm.vel = ctx->d_vel2;
   lm.acc = ctx->d_acc;
   lm.rec = recmap(ctx);
   lm.tr = traces;
+  lm.ring = ring(ctx, RING_U);
+  lm.inj = inj;
+  lm.trmax = maxp(ctx, MAX_TR);
+  int step = 0;  // adjoint steps taken (indexes the ring of nu)
   for (int sg = S - 1; sg >= 0; sg--) {
     const int k0 = sg * L, k1 = std::min(nt, k0 + L);
     if (sg < S - 1) {
       if (sg == 0) {
         CHK(zero(ctx, s_old, n));
         CHK(zero(ctx, s_cur, n));
+        if (half) CHK(ring_reset(ctx, RING_S));
       } else {
-        const float *ck = ctx->d_ckpt + (long long)2 * (sg - 1) * n;
-        HIPCHK(hipMemcpyAsync(s_cur, ck, Fb, hipMemcpyDeviceToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(s_old, ck + n, Fb, hipMemcpyDeviceToDevice, ctx->stream));
+        const long long c = sg - 1;
+        HIPCHK(hipMemcpyAsync(s_cur, nth(ctx->d_ckpt, 2 * c, n, esz), Fb,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(s_old, nth(ctx->d_ckpt, 2 * c + 1, n, esz), Fb,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        if (half)
+          HIPCHK(hipMemcpyAsync(ctx->d_ring + RING_S, ctx->d_ckring + 8 * c, 8 * sizeof(float),
+                                hipMemcpyDeviceToDevice, ctx->stream));
       }
       for (int k = k0; k < k1; k++) {
         sr.s1 = s_cur;
         sr.s0 = s_old;
-        sr.sig = ctx->d_seg + (long long)(k - k0) * n;
+        sr.sig = nth(ctx->d_seg, k - k0, n, esz);
+        sr.hscale = half ? ctx->d_hscale + (k - k0) : nullptr;
         sr.amp = ctx->d_amp + (long long)k * ctx->npts_s;
+        sr.h = half_state(ctx, k);
         HIPCHK(born::launch_srec_step(geo, stu, sr, ctx->stream));
         std::swap(s_old, s_cur);
         recomputed++;
       }
     }
-    const float *seg = ctx->d_seg;
     for (int k = k1 - 1; k >= k0; k--) {
       const int id = k / jt, ii = k - id * jt;
       lm.u1 = u_cur;
       lm.u0 = u_old;
-      lm.sig = seg + (long long)(k - k0) * n;
+      lm.sig = nth(ctx->d_seg, k - k0, n, esz);
+      lm.hscale = half ? ctx->d_hscale + (k - k0) : nullptr;
       lm.tab = ctx->d_tab_d + (long long)ii * 8;
       lm.id = id;
       lm.inject = id + 7 < ctx->nt_rec;
+      lm.h = half_state(ctx, step++);
       HIPCHK(born::launch_lam_step(geo, ltu, lm, ctx->stream));
       std::swap(u_old, u_cur);
     }
@@ -1022,6 +1173,15 @@ int born_last_checkpoint_plan(const born_ctx *ctx, int *segments, int *recompute
 int born_set_zchunk(born_ctx *ctx, int zchunk) {
   if (!ctx || zchunk < 0) return BORN_ERR_ARG;
   ctx->zchunk = zchunk;
+  return BORN_OK;
+}
+
+int born_set_storage(born_ctx *ctx, int storage) {
+  if (!ctx) return BORN_ERR_ARG;
+  if (storage != BORN_STORAGE_F32 && storage != BORN_STORAGE_F16)
+    return fail(ctx, BORN_ERR_ARG, "storage must be BORN_STORAGE_F32 or BORN_STORAGE_F16");
+  if (storage != ctx->storage) ctx->hist_version = -1;
+  ctx->storage = storage;
   return BORN_OK;
 }
 

@@ -16,6 +16,7 @@
 // Ops are passed by value as kernel arguments and never written, so their
 // pointers stay in the global address space (global_load/store, not flat).
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -64,11 +65,57 @@ struct RecMap {
 // element at 32-bit element offset o (fields are < 2^30 cells)
 template <class T>
 __device__ __forceinline__ T &at(T *p, uint32_t o) {
-  return *(T *)((char *)p + (uint64_t)(o << 2));
+  return *(T *)((char *)p + (uint64_t)(o * (uint32_t)sizeof(T)));
 }
 template <class T>
 __device__ __forceinline__ const T &at(const T *p, uint32_t o) {
-  return *(const T *)((const char *)p + (uint64_t)(o << 2));
+  return *(const T *)((const char *)p + (uint64_t)(o * (uint32_t)sizeof(T)));
+}
+
+// ---- storage of wavefield levels: float, or FP16 with a power-of-two scale
+// per stored level (value = half * inv_scale).  Conversions saturate at the
+// FP16 range instead of overflowing.
+__device__ __forceinline__ float ld_level(const float *p, uint32_t o, float) {
+  return at(p, o);
+}
+__device__ __forceinline__ float ld_level(const __half *p, uint32_t o, float inv) {
+  return __half2float(at(p, o)) * inv;
+}
+__device__ __forceinline__ void st_level(float *p, uint32_t o, float v, float) {
+  at(p, o) = v;
+}
+__device__ __forceinline__ void st_level(__half *p, uint32_t o, float v, float scale) {
+  at(p, o) = __float2half_rn(fminf(fmaxf(v * scale, -65504.f), 65504.f));
+}
+
+// Scale for a level whose magnitude is at most bound: a power of two that
+// maps bound to [8192, 16384), 4x below the FP16 maximum.  Relative precision
+// of FP16 does not depend on the scale; the scale only keeps the level inside
+// the normal range (a value 2^28 below the bound is still normal).
+__device__ __forceinline__ float level_scale(float bound) {
+  if (!(bound > 0.f)) return 1.f;
+  int e;
+  frexpf(bound, &e);  // bound < 2^e
+  e = min(max(14 - e, -120), 120);
+  return ldexpf(1.f, e);
+}
+
+// Scales of the stored levels of one FP16 field.  Level k lives in slot k % 4:
+// scale[s] (stored = value * scale) and maxb[s], the largest |value| of the
+// level as float bits (non-negative floats order like their bits).  The step
+// that reads levels k (cur) and k-1 (old) and writes k+1 (nxt) derives the
+// scale of k+1 from a bound on |level k+1| (gain * max k + max k-1 +
+// injections), accumulates max |level k+1| with atomics, and clears slot
+// k+2, which the following step fills.
+struct Ring {
+  float *scale;
+  unsigned *maxb;
+};
+struct Slots {
+  int cur, old, nxt, clr;
+};
+__device__ __forceinline__ float ring_max(const Ring &r, int s) {
+  return __uint_as_float(r.maxb[s]);
 }
 
 // The hardware hands block h to XCD h % 8.  Give every XCD a contiguous range
@@ -130,16 +177,18 @@ struct CellArgs {
 };
 
 // Generic z-marching leapfrog kernel.  Op supplies (all const):
-//   static constexpr int NF, NPRE; static constexpr bool DAMP;
-//   const float *cur(int f);                 current level of field f
+//   static constexpr int NF, NPRE, NMAX; static constexpr bool DAMP;
 //   Blk block(x0, x1, y0, y1, z0, z1);       uniform per-block state
+//   float ld(const Blk&, int f, uint32_t o);  current level of field f
 //   Pln plane(const Blk&, int z);            uniform per-plane state
-//   void prefetch(float (&pre)[NPRE], uint32_t o);   pointwise inputs
-//   void cell(const Pln&, const CellArgs&, const float *lap,
-//             const float *ctr, const float (&pre)[NPRE]);
+//   void prefetch(const Blk&, float (&pre)[NPRE], uint32_t o);
+//   void cell(const Blk&, const Pln&, const CellArgs&, const float *lap,
+//             const float *ctr, const float (&pre)[NPRE], float (&tmax)[NF]);
+//   void first(const Blk&);                  (NMAX > 0) once per launch
+//   void publish_max(int f, float m);        (NMAX > 0) block max of |new|
 template <class Op, int RY, int WY>
-__global__ void __launch_bounds__(kTX *WY)
-    march_kernel(const Grid g, const Taps t, const Sponge sp, const Op op) {
+__device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const Sponge &sp,
+                                           const Op &op, const int L) {
   constexpr int NF = Op::NF;
   constexpr int NP = Op::NPRE > 0 ? Op::NPRE : 1;
   constexpr int TY = RY * WY;
@@ -148,7 +197,6 @@ __global__ void __launch_bounds__(kTX *WY)
   constexpr int NXH = 2 * kH * TY;  // x-halo cells of a plane
   __shared__ float tile[NF][LH][LW];
 
-  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const int bx = L % g.ntx;
   const int rest = L / g.ntx;
   const int by = rest % g.nty;
@@ -166,6 +214,8 @@ __global__ void __launch_bounds__(kTX *WY)
 
   const auto blk = op.block(max(x0, kH), min(x0 + kTX, g.nx - kH), y0,
                             min(y0 + TY, g.ny - kH), z0, z1);
+  if constexpr (Op::NMAX > 0)
+    if (L == 0 && threadIdx.x == 0) op.first(blk);
 
   int yy[RY];
   bool in[RY];
@@ -214,22 +264,23 @@ __global__ void __launch_bounds__(kTX *WY)
   float lead[NF][RY];
   float halo[NF][HPT];
   float pre[RY][NP], pre_n[RY][NP];
+  float tmax[NF];
 #pragma unroll
   for (int f = 0; f < NF; f++) {
-    const float *p = op.cur(f);
+    tmax[f] = 0.f;
 #pragma unroll
     for (int j = 0; j < RY; j++) {
 #pragma unroll
       for (int k = 0; k < 2 * kH; k++)
-        q[f][j][k] = at(p, (uint32_t)(z0 - kH + k) * g.plane + row[j]);
-      lead[f][j] = at(p, (uint32_t)(z0 + kH) * g.plane + row[j]);
+        q[f][j][k] = op.ld(blk, f, (uint32_t)(z0 - kH + k) * g.plane + row[j]);
+      lead[f][j] = op.ld(blk, f, (uint32_t)(z0 + kH) * g.plane + row[j]);
     }
 #pragma unroll
     for (int k = 0; k < HPT; k++)
-      halo[f][k] = hv[k] ? at(p, (uint32_t)z0 * g.plane + hoff[k]) : 0.f;
+      halo[f][k] = hv[k] ? op.ld(blk, f, (uint32_t)z0 * g.plane + hoff[k]) : 0.f;
   }
 #pragma unroll
-  for (int j = 0; j < RY; j++) op.prefetch(pre[j], (uint32_t)z0 * g.plane + row[j]);
+  for (int j = 0; j < RY; j++) op.prefetch(blk, pre[j], (uint32_t)z0 * g.plane + row[j]);
 
   for (int z = z0; z < z1; ++z) {
     const uint32_t zo = (uint32_t)z * g.plane;
@@ -249,15 +300,14 @@ __global__ void __launch_bounds__(kTX *WY)
       const uint32_t zn = zo + g.plane;
 #pragma unroll
       for (int f = 0; f < NF; f++) {
-        const float *p = op.cur(f);
 #pragma unroll
-        for (int j = 0; j < RY; j++) lead[f][j] = at(p, zn + kH * g.plane + row[j]);
+        for (int j = 0; j < RY; j++) lead[f][j] = op.ld(blk, f, zn + kH * g.plane + row[j]);
 #pragma unroll
         for (int k = 0; k < HPT; k++)
-          if (hv[k]) halo[f][k] = at(p, zn + hoff[k]);
+          if (hv[k]) halo[f][k] = op.ld(blk, f, zn + hoff[k]);
       }
 #pragma unroll
-      for (int j = 0; j < RY; j++) op.prefetch(pre_n[j], zn + row[j]);
+      for (int j = 0; j < RY; j++) op.prefetch(blk, pre_n[j], zn + row[j]);
     }
     __syncthreads();
     const auto pl = op.plane(blk, z);
@@ -288,7 +338,7 @@ __global__ void __launch_bounds__(kTX *WY)
       c.z = z;
       c.o = zo + row[j];
       c.b = fminf(bxy[j], bzv);
-      op.cell(pl, c, lap, ctr, pre[j]);
+      op.cell(blk, pl, c, lap, ctr, pre[j], tmax);
     }
     __syncthreads();
 #pragma unroll
@@ -302,6 +352,67 @@ __global__ void __launch_bounds__(kTX *WY)
 #pragma unroll
       for (int k = 0; k < NP; k++) pre[j][k] = pre_n[j][k];
   }
+  // block maxima of |new level| for the next step's FP16 scales
+  if constexpr (Op::NMAX > 0) {
+    __shared__ float red[NF][WY];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      float m = tmax[f];
+      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+      if (tx == 0) red[f][ty] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        float m = 0.f;
+        for (int w = 0; w < WY; w++) m = fmaxf(m, red[f][w]);
+        op.publish_max(f, m);
+      }
+    }
+  }
+}
+
+template <class Op, int RY, int WY>
+__global__ void __launch_bounds__(kTX *WY)
+    march_kernel(const Grid g, const Taps t, const Sponge sp, const Op op) {
+  march_tile<Op, RY, WY>(g, t, sp, op, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+}
+
+// Tiles of a batch of shots with their own grids: shot s owns the logical
+// tiles [tile0[s], tile0[s + 1]).
+constexpr int kMaxBatch = 64;
+struct BatchTiles {
+  int n;
+  int zc;
+  int tile0[kMaxBatch + 1];
+};
+
+// Tile grid of a domain for tile shape (RY, WY) and z chunk zc.
+__host__ __device__ inline Grid tile_grid(int nx, int ny, int nz, int ty, int zc) {
+  Grid g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.plane = (uint32_t)nx * ny;
+  g.ntx = (nx - kH + kTX - 1) / kTX;
+  g.nty = (ny - 2 * kH + ty - 1) / ty;
+  g.zc = zc;
+  g.ntz = (nz - 2 * kH + zc - 1) / zc;
+  return g;
+}
+
+// One launch advances every shot of a batch by one step.  BOp::shot(s, ty,
+// zc) returns {Grid, Sponge, Op} of shot s with the op's pointers moved to
+// the shot's part of the batch arrays.
+template <class BOp, int RY, int WY>
+__global__ void __launch_bounds__(kTX *WY)
+    march_batch_kernel(const BatchTiles bt, const Taps t, const BOp bop) {
+  const int Lg = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  int s = 0;
+  while (s + 1 < bt.n && Lg >= bt.tile0[s + 1]) s++;
+  const auto sh = bop.shot(s, RY * WY, bt.zc);
+  march_tile<typename BOp::Op, RY, WY>(sh.g, t, sh.sp, sh.op, Lg - bt.tile0[s]);
 }
 
 }  // namespace born

@@ -18,6 +18,8 @@
 // update (cpu_prop.cc:141-165 on p1); that value is only ever read back as the
 // "old" level of the next step, so it is recomputed there as fl(b * old)
 // instead of being written: same arithmetic, one HBM write less per field.
+#include <type_traits>
+
 #include "born_device.h"
 #include "born_launch.h"
 
@@ -37,44 +39,103 @@ __device__ __forceinline__ int rec_first(const RecMap &rm, int slot, int x, int 
   return rm.rmap[((long long)slot * ny + y) * nx + x];
 }
 
+// FP16 storage state shared by the ops (unused with float storage): the
+// bound on |new level| is gain * max|cur| + max|old| + injections, with
+// gain = 2 + max(V) * tapsum and tapsum >= |L u| / max|u|.
+template <class T>
+constexpr bool kHalf = std::is_same<T, __half>::value;
+
+struct Blk1 {  // one stored field
+  uint32_t pm;
+  float ic, io, sn, sh;  // 1/scale of cur and old, scale of new and of sigma
+};
+struct Blk2 {  // two stored fields
+  uint32_t pm;
+  float ic[2], io[2], sn[2], sh;
+};
+
+__device__ __forceinline__ float amp_sum(const Points &pts, const float *amp) {
+  float a = 0.f;
+  for (int i = 0; i < pts.n; i++) a += fabsf(amp[i]);
+  return a;
+}
+
 // ---------------------------------------------------------------- forward
-template <int SCATTER>
+template <int SCATTER, class T>
 struct FwdOp {
-  static constexpr int NF = 2, NPRE = 4;
+  static constexpr bool H = kHalf<T>;
+  static constexpr int NF = 2, NPRE = 4, NMAX = H ? 2 : 0;
   static constexpr bool DAMP = true;
-  const float *__restrict__ s1;
-  const float *__restrict__ r1;
-  float *__restrict__ s0;
-  float *__restrict__ r0;
+  const T *__restrict__ s1;
+  const T *__restrict__ r1;
+  T *__restrict__ s0;
+  T *__restrict__ r0;
   const float *__restrict__ vel;
   const float *__restrict__ model;
-  float *__restrict__ hist;  // optional scatter-operand history
+  T *__restrict__ hist;  // optional scatter-operand history
   Points pts;
   const float *__restrict__ amp;  // source amplitudes of this step [npts]
   RecMap rm;
   float *__restrict__ tr;  // traces, time-major [ntb][nrec]
   const float *__restrict__ tab;
   int id, extract, nx, ny;
+  Ring ring[2];  // FP16: s, r
+  Slots sl;
+  float gain, tapsum;
+  const unsigned *mmax;  // max |model| as float bits
+  float *hscale;         // FP16: scale of this step's history level
 
-  __device__ const float *cur(int f) const { return f == 0 ? s1 : r1; }
-  __device__ uint32_t block(int x0, int x1, int y0, int y1, int z0, int z1) const {
-    return points_in_block(pts, x0, x1, y0, y1, z0, z1);
+  __device__ Blk2 block(int x0, int x1, int y0, int y1, int z0, int z1) const {
+    Blk2 b;
+    b.pm = points_in_block(pts, x0, x1, y0, y1, z0, z1);
+    for (int f = 0; f < 2; f++) b.ic[f] = b.io[f] = b.sn[f] = 1.f;
+    b.sh = 1.f;
+    if constexpr (H) {
+      for (int f = 0; f < 2; f++) {
+        b.ic[f] = 1.f / ring[f].scale[sl.cur];
+        b.io[f] = 1.f / ring[f].scale[sl.old];
+      }
+      const float ms = ring_max(ring[0], sl.cur);
+      const float bs = gain * ms + ring_max(ring[0], sl.old) + amp_sum(pts, amp);
+      const float mm = __uint_as_float(*mmax);
+      const float sig = SCATTER == 0 ? bs : tapsum * ms;
+      const float br = gain * ring_max(ring[1], sl.cur) + ring_max(ring[1], sl.old) +
+                       (SCATTER == 0 ? 1e-6f * bs * mm : mm * sig);
+      b.sn[0] = level_scale(bs);
+      b.sn[1] = level_scale(br);
+      b.sh = SCATTER == 0 ? b.sn[0] : level_scale(sig);
+    }
+    return b;
   }
-  __device__ SrcPln plane(uint32_t blk, int z) const {
-    return {points_on_plane(pts, blk, z), rm.plane_slot[z]};
+  __device__ void first(const Blk2 &b) const {
+    for (int f = 0; f < 2; f++) {
+      ring[f].scale[sl.nxt] = b.sn[f];
+      ring[f].maxb[sl.clr] = 0u;
+    }
+    if (hist) *hscale = b.sh;
   }
-  __device__ void prefetch(float (&p)[NPRE], uint32_t o) const {
-    p[0] = at(s0, o);
-    p[1] = at(r0, o);
+  __device__ void publish_max(int f, float m) const {
+    atomicMax(&ring[f].maxb[sl.nxt], __float_as_uint(m));
+  }
+  __device__ float ld(const Blk2 &b, int f, uint32_t o) const {
+    return ld_level(f == 0 ? s1 : r1, o, b.ic[f]);
+  }
+  __device__ SrcPln plane(const Blk2 &b, int z) const {
+    return {points_on_plane(pts, b.pm, z), rm.plane_slot[z]};
+  }
+  __device__ void prefetch(const Blk2 &b, float (&p)[NPRE], uint32_t o) const {
+    p[0] = ld_level(s0, o, b.io[0]);
+    p[1] = ld_level(r0, o, b.io[1]);
     p[2] = at(vel, o);
     p[3] = at(model, o);
   }
-  __device__ void cell(const SrcPln &pl, const CellArgs &c, const float *lap,
-                       const float *ctr, const float (&p)[NPRE]) const {
-    const float v = p[2], b = c.b;
-    float sn = b * (v * lap[0] + ctr[0] + ctr[0] - b * p[0]);
+  __device__ void cell(const Blk2 &b, const SrcPln &pl, const CellArgs &c,
+                       const float *lap, const float *ctr, const float (&p)[NPRE],
+                       float (&tmax)[NF]) const {
+    const float v = p[2], bb = c.b;
+    float sn = bb * (v * lap[0] + ctr[0] + ctr[0] - bb * p[0]);
     if (pl.pm) sn = add_points(pts, pl.pm, c.x, c.y, sn, amp);
-    float rn = b * (v * lap[1] + ctr[1] + ctr[1] - b * p[1]);
+    float rn = bb * (v * lap[1] + ctr[1] + ctr[1] - bb * p[1]);
     float sig;
     if (SCATTER == 0) {
       // the reference's double literal: evaluated in double, rounded once
@@ -82,12 +143,16 @@ struct FwdOp {
       rn = (float)((double)rn + .000001 * (double)sn * (double)p[3]);
       sig = sn;
     } else {
-      sig = b * lap[0];
+      sig = bb * lap[0];
       rn = rn + p[3] * sig;
     }
-    at(s0, c.o) = sn;
-    at(r0, c.o) = rn;
-    if (hist) at(hist, c.o) = sig;
+    st_level(s0, c.o, sn, b.sn[0]);
+    st_level(r0, c.o, rn, b.sn[1]);
+    if (hist) st_level(hist, c.o, sig, b.sh);
+    if constexpr (H) {
+      tmax[0] = fmaxf(tmax[0], fabsf(sn));
+      tmax[1] = fmaxf(tmax[1], fabsf(rn));
+    }
     if (pl.slot >= 0 && extract) {
       for (int i = rec_first(rm, pl.slot, c.x, c.y, nx, ny); i >= 0; i = rm.rnext[i]) {
         float *q = tr + (long long)id * rm.nrec + i;
@@ -102,26 +167,26 @@ struct FwdOp {
 
 // ------------------------------------------------------------- sourceProp
 struct SrcOp {
-  static constexpr int NF = 1, NPRE = 2;
+  static constexpr int NF = 1, NPRE = 2, NMAX = 0;
   static constexpr bool DAMP = false;
   const float *__restrict__ s1;
   float *__restrict__ s0;
   const float *__restrict__ vel;
   Points pts;
   const float *__restrict__ amp;
-  __device__ const float *cur(int) const { return s1; }
   __device__ uint32_t block(int x0, int x1, int y0, int y1, int z0, int z1) const {
     return points_in_block(pts, x0, x1, y0, y1, z0, z1);
   }
+  __device__ float ld(uint32_t, int, uint32_t o) const { return at(s1, o); }
   __device__ uint32_t plane(uint32_t blk, int z) const {
     return points_on_plane(pts, blk, z);
   }
-  __device__ void prefetch(float (&p)[NPRE], uint32_t o) const {
+  __device__ void prefetch(uint32_t, float (&p)[NPRE], uint32_t o) const {
     p[0] = at(s0, o);
     p[1] = at(vel, o);
   }
-  __device__ void cell(uint32_t pm, const CellArgs &c, const float *lap, const float *ctr,
-                       const float (&p)[NPRE]) const {
+  __device__ void cell(uint32_t, uint32_t pm, const CellArgs &c, const float *lap,
+                       const float *ctr, const float (&p)[NPRE], float (&)[NF]) const {
     float sn = p[1] * lap[0] + ctr[0] + ctr[0] - p[0];
     if (pm) sn = add_points(pts, pm, c.x, c.y, sn, amp);
     at(s0, c.o) = sn;
@@ -130,7 +195,7 @@ struct SrcOp {
 
 // ------------------------------------------------------ reference adjoint
 struct RadjOp {
-  static constexpr int NF = 2, NPRE = 5;
+  static constexpr int NF = 2, NPRE = 5, NMAX = 0;
   static constexpr bool DAMP = true;
   const float *__restrict__ s1;
   const float *__restrict__ r1;
@@ -151,22 +216,22 @@ struct RadjOp {
   float sc;  // (float)dir / (float)jt
   int id, inject, nx, ny;
 
-  __device__ const float *cur(int f) const { return f == 0 ? s1 : r1; }
   __device__ uint32_t block(int x0, int x1, int y0, int y1, int z0, int z1) const {
     return pre ? points_in_block(pts, x0, x1, y0, y1, z0, z1) : 0u;
   }
+  __device__ float ld(uint32_t, int f, uint32_t o) const { return at(f == 0 ? s1 : r1, o); }
   __device__ SrcPln plane(uint32_t blk, int z) const {
     return {points_on_plane(pts, blk, z), rm.plane_slot[z]};
   }
-  __device__ void prefetch(float (&p)[NPRE], uint32_t o) const {
+  __device__ void prefetch(uint32_t, float (&p)[NPRE], uint32_t o) const {
     p[0] = at(s0, o);
     p[1] = at(r0, o);
     p[2] = at(vel1, o);
     p[3] = at(vel2, o);
     p[4] = at(img, o);
   }
-  __device__ void cell(const SrcPln &pl, const CellArgs &c, const float *lap,
-                       const float *ctr, const float (&p)[NPRE]) const {
+  __device__ void cell(uint32_t, const SrcPln &pl, const CellArgs &c, const float *lap,
+                       const float *ctr, const float (&p)[NPRE], float (&)[NF]) const {
     float sn;
     if (s_prop) {
       float so = p[0];
@@ -189,75 +254,215 @@ struct RadjOp {
 };
 
 // ------------------------------------- exact adjoint: background recompute
-template <int SCATTER>
+template <int SCATTER, class T>
 struct SrecOp {
-  static constexpr int NF = 1, NPRE = 2;
+  static constexpr bool H = kHalf<T>;
+  static constexpr int NF = 1, NPRE = 2, NMAX = H ? 1 : 0;
   static constexpr bool DAMP = true;
-  const float *__restrict__ s1;
-  float *__restrict__ s0;
+  const T *__restrict__ s1;
+  T *__restrict__ s0;
   const float *__restrict__ vel;
-  float *__restrict__ sig;  // scatter operand of this step (nullptr: none)
+  T *__restrict__ sig;  // scatter operand of this step (nullptr: none)
   Points pts;
   const float *__restrict__ amp;
-  __device__ const float *cur(int) const { return s1; }
-  __device__ uint32_t block(int x0, int x1, int y0, int y1, int z0, int z1) const {
-    return points_in_block(pts, x0, x1, y0, y1, z0, z1);
+  Ring ring;
+  Slots sl;
+  float gain, tapsum;
+  float *hscale;  // FP16: scale of this step's sigma
+
+  __device__ Blk1 block(int x0, int x1, int y0, int y1, int z0, int z1) const {
+    Blk1 b;
+    b.pm = points_in_block(pts, x0, x1, y0, y1, z0, z1);
+    b.ic = b.io = b.sn = b.sh = 1.f;
+    if constexpr (H) {
+      b.ic = 1.f / ring.scale[sl.cur];
+      b.io = 1.f / ring.scale[sl.old];
+      const float ms = ring_max(ring, sl.cur);
+      const float bs = gain * ms + ring_max(ring, sl.old) + amp_sum(pts, amp);
+      b.sn = level_scale(bs);
+      b.sh = SCATTER == 0 ? b.sn : level_scale(tapsum * ms);
+    }
+    return b;
   }
-  __device__ uint32_t plane(uint32_t blk, int z) const {
-    return points_on_plane(pts, blk, z);
+  __device__ void first(const Blk1 &b) const {
+    ring.scale[sl.nxt] = b.sn;
+    ring.maxb[sl.clr] = 0u;
+    if (sig) *hscale = b.sh;
   }
-  __device__ void prefetch(float (&p)[NPRE], uint32_t o) const {
-    p[0] = at(s0, o);
+  __device__ void publish_max(int, float m) const {
+    atomicMax(&ring.maxb[sl.nxt], __float_as_uint(m));
+  }
+  __device__ float ld(const Blk1 &b, int, uint32_t o) const { return ld_level(s1, o, b.ic); }
+  __device__ uint32_t plane(const Blk1 &b, int z) const {
+    return points_on_plane(pts, b.pm, z);
+  }
+  __device__ void prefetch(const Blk1 &b, float (&p)[NPRE], uint32_t o) const {
+    p[0] = ld_level(s0, o, b.io);
     p[1] = at(vel, o);
   }
-  __device__ void cell(uint32_t pm, const CellArgs &c, const float *lap, const float *ctr,
-                       const float (&p)[NPRE]) const {
-    const float b = c.b;
-    float sn = b * (p[1] * lap[0] + ctr[0] + ctr[0] - b * p[0]);
+  __device__ void cell(const Blk1 &b, uint32_t pm, const CellArgs &c, const float *lap,
+                       const float *ctr, const float (&p)[NPRE], float (&tmax)[NF]) const {
+    const float bb = c.b;
+    float sn = bb * (p[1] * lap[0] + ctr[0] + ctr[0] - bb * p[0]);
     if (pm) sn = add_points(pts, pm, c.x, c.y, sn, amp);
-    at(s0, c.o) = sn;
-    if (sig) at(sig, c.o) = SCATTER == 0 ? sn : b * lap[0];
+    st_level(s0, c.o, sn, b.sn);
+    if constexpr (H) tmax[0] = fmaxf(tmax[0], fabsf(sn));
+    if (sig) st_level(sig, c.o, SCATTER == 0 ? sn : bb * lap[0], b.sh);
   }
 };
 
 // ---------------------------------------- exact adjoint: adjoint-state step
 // nu = V B lambda obeys the forward recursion with V B-weighted receiver
 // injection; the image accumulates sigma * nu and is divided by V B at the end.
+template <class T>
 struct LamOp {
-  static constexpr int NF = 1, NPRE = 4;
+  static constexpr bool H = kHalf<T>;
+  static constexpr int NF = 1, NPRE = 4, NMAX = H ? 1 : 0;
   static constexpr bool DAMP = true;
-  const float *__restrict__ u1;
-  float *__restrict__ u0;
+  const T *__restrict__ u1;
+  T *__restrict__ u0;
   const float *__restrict__ vel;
-  const float *__restrict__ sig;
+  const T *__restrict__ sig;
   float *__restrict__ acc;
   RecMap rm;
   const float *__restrict__ tr;
   const float *__restrict__ tab;
   int id, inject, nx, ny;
-  __device__ const float *cur(int) const { return u1; }
-  __device__ int block(int, int, int, int, int, int) const { return 0; }
-  __device__ int plane(int, int z) const { return rm.plane_slot[z]; }
-  __device__ void prefetch(float (&p)[NPRE], uint32_t o) const {
-    p[0] = at(u0, o);
+  Ring ring;
+  Slots sl;
+  float gain;
+  float inj;                // bound of |V B P^T w| per unit of max |trace|
+  const unsigned *trmax;    // max |trace| as float bits
+  const float *hscale;      // FP16: scale of sigma
+
+  __device__ Blk1 block(int, int, int, int, int, int) const {
+    Blk1 b;
+    b.pm = 0;
+    b.ic = b.io = b.sn = b.sh = 1.f;
+    if constexpr (H) {
+      b.ic = 1.f / ring.scale[sl.cur];
+      b.io = 1.f / ring.scale[sl.old];
+      const float bu = gain * ring_max(ring, sl.cur) + ring_max(ring, sl.old) +
+                       (inject ? inj * __uint_as_float(*trmax) : 0.f);
+      b.sn = level_scale(bu);
+      b.sh = 1.f / *hscale;
+    }
+    return b;
+  }
+  __device__ void first(const Blk1 &b) const {
+    ring.scale[sl.nxt] = b.sn;
+    ring.maxb[sl.clr] = 0u;
+  }
+  __device__ void publish_max(int, float m) const {
+    atomicMax(&ring.maxb[sl.nxt], __float_as_uint(m));
+  }
+  __device__ float ld(const Blk1 &b, int, uint32_t o) const { return ld_level(u1, o, b.ic); }
+  __device__ int plane(const Blk1 &, int z) const { return rm.plane_slot[z]; }
+  __device__ void prefetch(const Blk1 &b, float (&p)[NPRE], uint32_t o) const {
+    p[0] = ld_level(u0, o, b.io);
     p[1] = at(vel, o);
-    p[2] = at(sig, o);
+    p[2] = ld_level(sig, o, b.sh);
     p[3] = at(acc, o);
   }
-  __device__ void cell(int slot, const CellArgs &c, const float *lap, const float *ctr,
-                       const float (&p)[NPRE]) const {
-    const float b = c.b, v = p[1];
-    float un = b * (v * lap[0] + ctr[0] + ctr[0] - b * p[0]);
+  __device__ void cell(const Blk1 &b, int slot, const CellArgs &c, const float *lap,
+                       const float *ctr, const float (&p)[NPRE], float (&tmax)[NF]) const {
+    const float bb = c.b, v = p[1];
+    float un = bb * (v * lap[0] + ctr[0] + ctr[0] - bb * p[0]);
     if (slot >= 0 && inject) {
       float g = 0.f;
       for (int i = rec_first(rm, slot, c.x, c.y, nx, ny); i >= 0; i = rm.rnext[i])
         g += sinc8(tab, tr + (long long)id * rm.nrec + i, rm.nrec);
-      un += v * b * g;
+      un += v * bb * g;
     }
-    at(u0, c.o) = un;
+    st_level(u0, c.o, un, b.sn);
+    if constexpr (H) tmax[0] = fmaxf(tmax[0], fabsf(un));
     at(acc, c.o) = p[3] + p[2] * un;
   }
 };
+
+// ------------------------------------------------------------ shot batches
+// The forward op of every shot of a batch: the op's arrays start at the
+// batch's first shot and are moved by the shot's descriptor.
+template <int SCATTER, class T>
+struct BatchFwd {
+  using Op = FwdOp<SCATTER, T>;
+  Op base;
+  const float *sponge;
+  float *rings;
+  const ShotDesc *__restrict__ desc;
+  int step;
+  struct Shot {
+    Grid g;
+    Sponge sp;
+    Op op;
+  };
+  __device__ Shot shot(int s, int ty, int zc) const {
+    const ShotDesc d = desc[s];
+    Shot r;
+    r.g = tile_grid(d.nx, d.ny, d.nz, ty, zc);
+    r.sp.bx = sponge + d.sp0;
+    r.sp.by = sponge + d.sp0 + d.nx;
+    r.sp.bz = sponge + d.sp0 + d.nx + d.ny;
+    Op &o = r.op;
+    o = base;
+    o.s1 = base.s1 + d.cell0;
+    o.r1 = base.r1 + d.cell0;
+    o.s0 = base.s0 + d.cell0;
+    o.r0 = base.r0 + d.cell0;
+    o.vel = base.vel + d.cell0;
+    o.model = base.model + d.cell0;
+    o.hist = nullptr;
+    o.pts.p = base.pts.p + d.pts0;
+    o.pts.n = d.npts;
+    o.amp = base.amp + d.amp0 + (long long)step * d.npts;
+    o.rm.plane_slot = base.rm.plane_slot + d.slot0;
+    o.rm.rmap = base.rm.rmap + d.rmap0;
+    o.rm.rnext = base.rm.rnext + d.rnext0;
+    o.rm.nrec = d.nrec;
+    o.tr = base.tr + d.tr0;
+    o.nx = d.nx;
+    o.ny = d.ny;
+    float *rg = rings + d.ring0;
+    o.ring[0] = {rg, (unsigned *)(rg + 4)};
+    o.ring[1] = {rg + 8, (unsigned *)(rg + 12)};
+    o.mmax = (const unsigned *)(rg + 16);
+    o.gain = d.gain;
+    return r;
+  }
+};
+
+// per shot: both rings at scale 1, max 0, and max |model| 0
+__global__ void batch_ring_init_kernel(float *rings, int nshots) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshots * kRingWords) return;
+  const int w = i % kRingWords;
+  rings[i] = (w < 4 || (w >= 8 && w < 12)) ? 1.f : 0.f;
+}
+
+// ring slots: scale 1, max 0
+__global__ void ring_init_kernel(float *scale, unsigned *maxb, int n) {
+  const int i = threadIdx.x;
+  if (i < n) {
+    scale[i] = 1.f;
+    maxb[i] = 0u;
+  }
+}
+
+// *out = max(*out, max |p[i]|) as float bits (*out starts at 0)
+__global__ void absmax_kernel(const float *p, long long n, unsigned *out) {
+  float m = 0.f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(p[i]));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  __shared__ float red[4];
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)blockDim.x / 64; w++) m = fmaxf(m, red[w]);
+    atomicMax(out, __float_as_uint(m));
+  }
+}
 
 // ------------------------------------------------------------ small kernels
 
@@ -320,17 +525,7 @@ __global__ void transpose_kernel(float *dst, const float *src, int n0, int n1) {
 const TileShape kShapes[kNumShapes] = {{2, 4}, {4, 4}, {2, 8}, {2, 16}, {4, 8}, {3, 8}};
 
 static Grid make_grid(const Geometry &geo, const TileShape &ts, int zc) {
-  Grid g;
-  g.nx = geo.nx;
-  g.ny = geo.ny;
-  g.nz = geo.nz;
-  g.plane = (uint32_t)geo.nx * geo.ny;
-  const int ty = ts.ry * ts.wy;
-  g.ntx = (geo.nx - kH + kTX - 1) / kTX;
-  g.nty = (geo.ny - 2 * kH + ty - 1) / ty;
-  g.zc = zc > 0 ? zc : 32;
-  g.ntz = (geo.nz - 2 * kH + g.zc - 1) / g.zc;
-  return g;
+  return tile_grid(geo.nx, geo.ny, geo.nz, ts.ry * ts.wy, zc > 0 ? zc : 32);
 }
 
 template <class Op, int RY, int WY>
@@ -357,23 +552,38 @@ static hipError_t launch(const Geometry &geo, const Tune &tu, const Op &op, hipS
 
 static Points to_points(const SrcPts &s) { return {(const int4 *)s.p, s.n}; }
 
+template <class T>
+static T *as(const void *p) {
+  return (T *)p;
+}
+
 hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a,
                            hipStream_t st) {
-  auto fill = [&](auto &op) {
-    op.s1 = a.s1; op.r1 = a.r1; op.s0 = a.s0; op.r0 = a.r0;
-    op.vel = a.vel; op.model = a.model; op.hist = a.hist;
+  auto fill = [&](auto &op, auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    op.s1 = as<T>(a.s1); op.r1 = as<T>(a.r1); op.s0 = as<T>(a.s0); op.r0 = as<T>(a.r0);
+    op.vel = a.vel; op.model = a.model; op.hist = as<T>(a.hist);
     op.pts = to_points(a.src); op.amp = a.amp;
     op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
     op.extract = a.extract; op.nx = geo.nx; op.ny = geo.ny;
-  };
-  if (a.scatter == 0) {
-    FwdOp<0> op{};
-    fill(op);
+    op.ring[0] = a.ring[0]; op.ring[1] = a.ring[1]; op.sl = a.h.sl;
+    op.gain = a.h.gain; op.tapsum = a.h.tapsum; op.mmax = a.mmax; op.hscale = a.hscale;
     return launch(geo, tu, op, st);
+  };
+  if (a.h.on) {
+    if (a.scatter == 0) {
+      FwdOp<0, __half> op{};
+      return fill(op, (__half *)nullptr);
+    }
+    FwdOp<1, __half> op{};
+    return fill(op, (__half *)nullptr);
   }
-  FwdOp<1> op{};
-  fill(op);
-  return launch(geo, tu, op, st);
+  if (a.scatter == 0) {
+    FwdOp<0, float> op{};
+    return fill(op, (float *)nullptr);
+  }
+  FwdOp<1, float> op{};
+  return fill(op, (float *)nullptr);
 }
 
 hipError_t launch_src_step(const Geometry &geo, const Tune &tu, const SrcStep &a,
@@ -398,27 +608,47 @@ hipError_t launch_radj_step(const Geometry &geo, const Tune &tu, const RadjStep 
 
 hipError_t launch_srec_step(const Geometry &geo, const Tune &tu, const SrecStep &a,
                             hipStream_t st) {
-  auto fill = [&](auto &op) {
-    op.s1 = a.s1; op.s0 = a.s0; op.vel = a.vel; op.sig = a.sig;
+  auto fill = [&](auto &op, auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    op.s1 = as<T>(a.s1); op.s0 = as<T>(a.s0); op.vel = a.vel; op.sig = as<T>(a.sig);
     op.pts = to_points(a.src); op.amp = a.amp;
-  };
-  if (a.scatter == 0) {
-    SrecOp<0> op{};
-    fill(op);
+    op.ring = a.ring; op.sl = a.h.sl; op.gain = a.h.gain; op.tapsum = a.h.tapsum;
+    op.hscale = a.hscale;
     return launch(geo, tu, op, st);
+  };
+  if (a.h.on) {
+    if (a.scatter == 0) {
+      SrecOp<0, __half> op{};
+      return fill(op, (__half *)nullptr);
+    }
+    SrecOp<1, __half> op{};
+    return fill(op, (__half *)nullptr);
   }
-  SrecOp<1> op{};
-  fill(op);
-  return launch(geo, tu, op, st);
+  if (a.scatter == 0) {
+    SrecOp<0, float> op{};
+    return fill(op, (float *)nullptr);
+  }
+  SrecOp<1, float> op{};
+  return fill(op, (float *)nullptr);
 }
 
 hipError_t launch_lam_step(const Geometry &geo, const Tune &tu, const LamStep &a,
                            hipStream_t st) {
-  LamOp op{};
-  op.u1 = a.u1; op.u0 = a.u0; op.vel = a.vel; op.sig = a.sig; op.acc = a.acc;
-  op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
-  op.inject = a.inject; op.nx = geo.nx; op.ny = geo.ny;
-  return launch(geo, tu, op, st);
+  auto fill = [&](auto &op, auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    op.u1 = as<T>(a.u1); op.u0 = as<T>(a.u0); op.vel = a.vel; op.sig = as<T>(a.sig);
+    op.acc = a.acc; op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
+    op.inject = a.inject; op.nx = geo.nx; op.ny = geo.ny;
+    op.ring = a.ring; op.sl = a.h.sl; op.gain = a.h.gain; op.inj = a.inj;
+    op.trmax = a.trmax; op.hscale = a.hscale;
+    return launch(geo, tu, op, st);
+  };
+  if (a.h.on) {
+    LamOp<__half> op{};
+    return fill(op, (__half *)nullptr);
+  }
+  LamOp<float> op{};
+  return fill(op, (float *)nullptr);
 }
 
 hipError_t launch_fill(float *p, long long n, float v, hipStream_t st) {
@@ -446,6 +676,93 @@ hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
   if (nb > 65536) nb = 65536;
   hipLaunchKernelGGL(adj_finish_kernel, dim3((unsigned)nb), dim3(256), 0, st, out, acc, vel,
                      geo.sponge, geo.nx, geo.ny, geo.nz, scale);
+  return hipGetLastError();
+}
+
+template <class BOp, int RY, int WY>
+static hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdStep &b,
+                                     const BOp &op, hipStream_t st) {
+  if (b.nshots < 1 || b.nshots > kMaxBatch) return hipErrorInvalidValue;
+  BatchTiles bt{};
+  bt.n = b.nshots;
+  bt.zc = zc > 0 ? zc : 32;
+  int tiles = 0;
+  for (int s = 0; s < b.nshots; s++) {
+    const ShotDesc &d = b.host_desc[s];
+    const Grid g = tile_grid(d.nx, d.ny, d.nz, RY * WY, bt.zc);
+    bt.tile0[s] = tiles;
+    tiles += g.ntx * g.nty * g.ntz;
+  }
+  bt.tile0[b.nshots] = tiles;
+  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY>), dim3(tiles), dim3(kTX * WY), 0, st,
+                     bt, taps, op);
+  return hipGetLastError();
+}
+
+template <class BOp>
+static hipError_t launch_batch(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
+                               const BOp &op, hipStream_t st) {
+  switch (tu.shape) {
+    case 0: return launch_batch_shape<BOp, 2, 4>(taps, tu.zchunk, b, op, st);
+    case 1: return launch_batch_shape<BOp, 4, 4>(taps, tu.zchunk, b, op, st);
+    case 2: return launch_batch_shape<BOp, 2, 8>(taps, tu.zchunk, b, op, st);
+    case 3: return launch_batch_shape<BOp, 2, 16>(taps, tu.zchunk, b, op, st);
+    case 4: return launch_batch_shape<BOp, 4, 8>(taps, tu.zchunk, b, op, st);
+    case 5: return launch_batch_shape<BOp, 3, 8>(taps, tu.zchunk, b, op, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
+                                 hipStream_t st) {
+  const FwdStep &a = b.a;
+  auto go = [&](auto &bop, auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    auto &op = bop.base;
+    op.s1 = as<T>(a.s1); op.r1 = as<T>(a.r1); op.s0 = as<T>(a.s0); op.r0 = as<T>(a.r0);
+    op.vel = a.vel; op.model = a.model; op.hist = nullptr;
+    op.pts = to_points(a.src); op.amp = a.amp;
+    op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
+    op.extract = a.extract;
+    op.sl = a.h.sl; op.tapsum = a.h.tapsum; op.hscale = nullptr;
+    bop.sponge = b.sponge; bop.rings = b.rings; bop.desc = b.desc; bop.step = b.step;
+    return launch_batch(taps, tu, b, bop, st);
+  };
+  if (a.h.on) {
+    if (a.scatter == 0) {
+      BatchFwd<0, __half> op{};
+      return go(op, (__half *)nullptr);
+    }
+    BatchFwd<1, __half> op{};
+    return go(op, (__half *)nullptr);
+  }
+  if (a.scatter == 0) {
+    BatchFwd<0, float> op{};
+    return go(op, (float *)nullptr);
+  }
+  BatchFwd<1, float> op{};
+  return go(op, (float *)nullptr);
+}
+
+hipError_t launch_batch_ring_init(float *rings, int nshots, hipStream_t st) {
+  const int n = nshots * kRingWords;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_ring_init_kernel, dim3((n + 255) / 256), dim3(256), 0, st, rings,
+                     nshots);
+  return hipGetLastError();
+}
+
+hipError_t launch_ring_init(float *scale, unsigned *maxb, int n, hipStream_t st) {
+  if (n > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(ring_init_kernel, dim3(1), dim3(64), 0, st, scale, maxb, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_absmax(const float *p, long long n, unsigned *out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long nb = (n + 255) / 256;
+  if (nb > 2048) nb = 2048;
+  hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)nb), dim3(256), 0, st, p, n, out);
   return hipGetLastError();
 }
 

@@ -33,6 +33,14 @@ struct SrcPts {
   int n;
 };
 
+// FP16 storage of the step's wavefield levels (and of the history): the field
+// pointers then address __half arrays; see Ring in born_device.h.
+struct Half {
+  int on;
+  Slots sl;
+  float gain, tapsum;
+};
+
 struct FwdStep {
   const float *s1, *r1;
   float *s0, *r0;
@@ -44,6 +52,10 @@ struct FwdStep {
   float *tr;
   const float *tab;
   int id, extract, scatter;
+  Half h;
+  Ring ring[2];
+  const unsigned *mmax;
+  float *hscale;
 };
 
 struct SrcStep {
@@ -78,6 +90,9 @@ struct SrecStep {
   SrcPts src;
   const float *amp;
   int scatter;
+  Half h;
+  Ring ring;
+  float *hscale;
 };
 
 struct LamStep {
@@ -89,10 +104,49 @@ struct LamStep {
   const float *tr;
   const float *tab;
   int id, inject;
+  Half h;
+  Ring ring;
+  float inj;
+  const unsigned *trmax;
+  const float *hscale;
+};
+
+// Placement of one shot of a batch inside the batch's arrays (elements).
+struct ShotDesc {
+  int nx, ny, nz;
+  int npts;
+  long long cell0;  // fields, velocity and model
+  long long rmap0;  // receiver map
+  long long tr0;    // traces, time-major [ntb][nrec]
+  long long amp0;   // source amplitudes [nt][npts]
+  int sp0;          // sponge tables: x at sp0, y at sp0 + nx, z at sp0 + nx + ny
+  int pts0;         // source points
+  int slot0;        // receiver plane slots (nz entries)
+  int rnext0;       // receiver chains
+  int nrec;
+  int ring0;        // FP16 scales: ring s (8 words), ring r (8), max |model| (1)
+  float gain;       // FP16 bound gain: 2 + max(V) * tapsum
+  int pad;
+};
+constexpr int kRingWords = 32;  // ring words per shot
+
+// One step of a batch: a.* hold the arrays of the whole batch (shot 0 at
+// offset 0), moved per shot by the descriptors.
+struct BatchFwdStep {
+  FwdStep a;
+  const float *sponge;
+  float *rings;
+  const ShotDesc *desc;      // device copy
+  const ShotDesc *host_desc; // host copy (tile counts)
+  int nshots;
+  int step;
 };
 
 hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a,
                            hipStream_t st);
+hipError_t launch_batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
+                                 hipStream_t st);
+hipError_t launch_batch_ring_init(float *rings, int nshots, hipStream_t st);
 hipError_t launch_src_step(const Geometry &geo, const Tune &tu, const SrcStep &a,
                            hipStream_t st);
 hipError_t launch_radj_step(const Geometry &geo, const Tune &tu, const RadjStep &a,
@@ -108,5 +162,7 @@ hipError_t launch_src_amp(float *amp, int nsteps, int npts, const float *src, in
 hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
                              const float *vel, float scale, hipStream_t st);
 hipError_t launch_transpose(float *dst, const float *src, int n0, int n1, hipStream_t st);
+hipError_t launch_ring_init(float *scale, unsigned *maxb, int n, hipStream_t st);
+hipError_t launch_absmax(const float *p, long long n, unsigned *out, hipStream_t st);
 
 }  // namespace born

@@ -8,6 +8,11 @@ library's stream.
 
 There is no fallback: importing works anywhere, but constructing a
 Propagator without the built library or without a GPU raises.
+
+A process that also uses torch on the GPU must import torch before the
+library is loaded: torch ships its own libamdhip64, and loading it second
+would bring a second HIP runtime into the process, which then sees no
+device.  Loaded after torch, the library binds to torch's runtime.
 """
 import ctypes as C
 import os
@@ -50,6 +55,7 @@ SIGNATURES = {
     "born_set_checkpoint_budget": (_i, [_vp, C.c_size_t]),
     "born_last_checkpoint_plan": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "born_set_zchunk": (_i, [_vp, _i]),
+    "born_set_storage": (_i, [_vp, _i]),
     "born_set_autotune": (_i, [_vp, _i]),
     "born_get_tuning": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
                              C.POINTER(C.c_float)]),
@@ -215,7 +221,14 @@ class Propagator:
         self._chk(self.lib.born_set_autotune(self.h, int(on)), "autotune")
 
     KERNELS = ("fwd", "fwd_dV", "source", "rtm_adjoint", "background", "background_dV",
-               "adjoint_state")
+               "adjoint_state", "fwd_f16", "fwd_dV_f16", "background_f16",
+               "background_dV_f16", "adjoint_state_f16")
+    STORAGE = {"f32": 0, "f16": 1}
+
+    def set_storage(self, kind):
+        """Wavefield storage of the Born forward / exact adjoint: "f32" or
+        "f16" (FP16 levels with per-level power-of-two scales)."""
+        self._chk(self.lib.born_set_storage(self.h, self.STORAGE[kind]), "storage")
 
     def tuning(self):
         out = {}

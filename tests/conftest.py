@@ -4,6 +4,14 @@ import sys
 
 import pytest
 
+# Load torch's HIP runtime before the propagator library: the library then
+# binds to the same libamdhip64 (one runtime per process), so tests that pass
+# torch device tensors to the device entry points can use both.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "sc17-born_amd")
 if ROOT not in sys.path:
