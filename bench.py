@@ -34,7 +34,7 @@ METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs F
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of this script
 # (profiles/born_fwd_adj_304cube_hbm_counters.md); reported as roofline.traffic
 # only when the measured workload and kernel are the profiled ones.
-PROFILED_TRAFFIC = {("c2", "FwdOp<0>, 2, 4", "f32"): 1301.5e6}
+PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 2, 4, 1", "f32"): 1301.5e6}
 
 
 def load(name):
@@ -93,6 +93,96 @@ def cpu_baseline(p, nthreads, target_s=15.0):
     }
 
 
+def batch_leg(args, pyborn, configs, torch):
+    """32 shots on windowed subgrids (BATCHES["c4"]), Born forward: the FP16
+    batch (one launch per step for all shots, time loop replayed from a HIP
+    graph) against the same shots run one by one in FP32, each on its own
+    context; the FP32 batch separates batching from storage."""
+    cfg = dict(configs.BATCHES["c4"])
+    if args.batch_nt:
+        cfg["nt"] = args.batch_nt
+    if args.batch_shots:
+        cfg["nshots"] = args.batch_shots
+    t0 = time.perf_counter()
+    shots, tab = configs.make_batch(**cfg)
+    nt = cfg["nt"]
+    log("batch problem built in %.1f s: %d shots" % (time.perf_counter() - t0, len(shots)))
+    dev = torch.device("cuda:0")
+    cells = sum(s.interior_cells for s in shots)
+    # FP32 one shot at a time
+    singles = []
+    for s in shots:
+        P = pyborn.Propagator(0)
+        P.create_space(10.0, 10.0, 10.0, s.nx, s.ny, s.nz)
+        P.transfer_sinc_table_s(tab)
+        P.transfer_sinc_table_d(tab)
+        P.transfer_vel_func1(s.V)
+        P.transfer_vel_func2(s.V)
+        P.transfer_receiver_func(s.rec_nx, s.rec_ny, s.rec_locs,
+                                 np.zeros((s.nrec, s.nt_big), np.float32))
+        P.transfer_source_func(s.src_locs, s.src_vals)
+        m = torch.from_numpy(s.model).to(dev)
+        tr = torch.zeros((s.nt_big, s.nrec), dtype=torch.float32, device=dev)
+        singles.append((P, m, tr))
+
+    def run_singles():
+        for P, m, tr in singles:
+            tr.zero_()
+            P.dev_forward(1, nt, m, tr, scatter=0, keep_background=False)
+        for P, _, _ in singles:
+            P.synchronize()
+
+    run_singles()  # autotune
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_singles()
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    log("batch leg: FP32 one shot at a time %.2f s" % t_single)
+    ref0 = singles[0][2].cpu().numpy().T.copy()
+    for P, m, tr in singles:
+        P.close()
+    del singles
+    torch.cuda.empty_cache()
+    # the batch
+    P = pyborn.Propagator(0)
+    P.transfer_sinc_table_s(tab)
+    P.transfer_sinc_table_d(tab)
+    B = pyborn.Batch(P, [{"nx": s.nx, "ny": s.ny, "nz": s.nz, "vel": s.V, "model": s.model,
+                          "src_locs": s.src_locs, "src_vals": s.src_vals, "rec_nx": s.rec_nx,
+                          "rec_ny": s.rec_ny, "rec_locs": s.rec_locs} for s in shots],
+                     (10.0, 10.0, 10.0))
+    out = {}
+    for storage in ("f32", "f16"):
+        P.set_storage(storage)
+        B.forward(1, nt, 0)  # autotune and graph capture
+        P.synchronize()
+        t0 = time.perf_counter()
+        B.forward(1, nt, 0)
+        P.synchronize()
+        out[storage] = time.perf_counter() - t0
+        log("batch leg: %s batch %.2f s" % (storage, out[storage]))
+    d16 = B.traces(0)
+    err = float(np.linalg.norm(d16 - ref0) / np.linalg.norm(ref0))
+    tile = B.tile(0)
+    B.close()
+    P.close()
+    mcu = lambda t: round(2 * cells * nt / t / 1e6, 1)
+    return {
+        "workload": "Born forward, %d shots on windows of [%d, %d]^2 x %d cells at 10 m "
+                    "(padded per shot), 96x96 receivers each, %d steps"
+                    % (len(shots), cfg["nxy"][0], cfg["nxy"][1], cfg["nz"], nt),
+        "interior_cells": cells,
+        "unit": "Mcell-updates/s (2 wavefield updates per cell-step)",
+        "f32_one_shot_at_a_time": mcu(t_single),
+        "f32_batch": mcu(out["f32"]),
+        "f16_batch": mcu(out["f16"]),
+        "f16_batch_speedup_vs_f32_plain": round(t_single / out["f16"], 3),
+        "f16_batch_tile": tile,
+        "f16_vs_f32_data_rel_l2_shot0": round(err, 6),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +196,9 @@ def main():
     ap.add_argument("--nt", type=int, default=0, help="time steps (profiling runs)")
     ap.add_argument("--shots", type=int, default=0, help="shots (profiling runs)")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-shots leg")
+    ap.add_argument("--batch-nt", type=int, default=0, help="time steps of the batch leg")
+    ap.add_argument("--batch-shots", type=int, default=0, help="shots of the batch leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (a kernel of known traffic "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -214,7 +307,8 @@ def main():
     achieved = fwd_alg_bytes / fwd_launch_s / 1e9
     alg_gbs = (FWD_BYTES_PER_CELL + ADJ_BYTES_PER_CELL) * cell_steps / elapsed / 1e9
     ft = tuning["fwd"]
-    fwd_kernel = "FwdOp<0>, %d, %d" % (ft["rows_per_thread"], ft["waves"])
+    fwd_kernel = "FwdOp<0, float>, %d, %d, %d" % (ft["rows_per_thread"], ft["waves"],
+                                                  ft["cells_per_lane"])
     traffic = PROFILED_TRAFFIC.get((args.config, fwd_kernel, "f32"))
     if args.nt or args.shots:
         traffic = None
@@ -273,6 +367,10 @@ def main():
             "bytes_per_born_cell_step": F16_BYTES_PER_CELL,
         }
     P.close()
+    del model, traces, img
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        out["config"]["batch"] = batch_leg(args, pyborn, configs, torch)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

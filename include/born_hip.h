@@ -173,7 +173,47 @@ int born_set_autotune(born_ctx *ctx, int on);
 int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
                     int *waves_per_block, int *zchunk, float *ms);
 
+/* The chosen tile of a step kernel in cells: tile_x = 64 * cells per lane
+ * (64 or 128), tile_y = rows per thread * waves per block. */
+int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y);
+/* Fix the launch shape of a step kernel (rows per thread x waves per block
+ * x cells per lane, one of the autotuner's shapes; 2 cells per lane needs
+ * an even nx) and its z chunk; the autotuner then leaves that kernel. */
+int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_per_block,
+                    int cells_per_lane, int zchunk);
+
 void *born_get_stream(const born_ctx *ctx);
+
+/* ---- Shot batches (no reference counterpart: the reference runs one shot
+ * per process, scripts/run.py).  Many shots, each with its own windowed
+ * domain, velocity, model, source and receivers, advance together: one
+ * launch per time step covers every shot's tiles, and the time loop is
+ * captured once in a HIP graph and replayed.  The spacing is shared; the
+ * sinc tables, sponge, storage (born_set_storage) and stream come from ctx,
+ * which must outlive the batch.  Every shot's nx * ny * nz < 2^30. */
+typedef struct born_batch born_batch;
+typedef struct {
+  int nx, ny, nz;        /* the shot's domain, x fastest */
+  const float *vel;      /* host [nz][ny][nx], V = v^2 dt^2 (as transfer_vel_func2) */
+  const float *model;    /* host [nz][ny][nx], the Born scatter operand */
+  int npts, nt_big;      /* source points (<= 32) and trace length (same for all) */
+  const int *src_locs;   /* linear cell index per source point */
+  const float *src_vals; /* [npts][nt_big] */
+  int rec_nx, rec_ny;    /* receivers */
+  const int *rec_locs;   /* linear cell index per receiver, rec_nx * rec_ny */
+} born_shot;
+int born_batch_create(born_ctx *ctx, int nshots, const born_shot *shots, float d1, float d2,
+                      float d3, born_batch **out);
+int born_batch_destroy(born_batch *b);
+/* Born forward of every shot (born_rtm_forward per shot: scatter 0 is the
+ * reference's term, 1 the dV term).  Queued on the context stream. */
+int born_batch_forward(born_batch *b, int jt, int nt, int scatter);
+/* Traces of one shot after born_batch_forward, [nrec][nt_big] host memory
+ * (synchronises the stream). */
+int born_batch_traces(born_batch *b, int shot, float *rec);
+/* Tile (cells) and z chunk the autotuner chose for the batch kernel. */
+int born_batch_get_tile(const born_batch *b, int scatter, int *tile_x, int *tile_y,
+                        int *zchunk, float *ms);
 int born_synchronize(born_ctx *ctx);
 
 #ifdef __cplusplus

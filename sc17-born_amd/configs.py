@@ -134,10 +134,82 @@ def make_problem(n=(256, 256, 256), d=10.0, nbound=20, nt=1500, dt=0.001, nshots
     return p
 
 
+class Shot:
+    """One shot of a batch: its own padded window, V = v^2 dt^2, model,
+    source stamp and receiver patch."""
+
+
+def make_batch(nshots=32, nxy=(200, 320), nz=300, d=10.0, nbound=20, nt=2000, dt=0.001,
+               rec=(96, 96), rec_d=10.0, rec_depth=10.0, src_depth=20.0, fpeak=12.0,
+               delay=0.1, nw=200, seed=1004):
+    """Shots on windowed subgrids: shot k's window is nx x ny x nz with nx, ny
+    drawn uniformly from nxy (seed + k), its own velocity law v(z) (seeded
+    surface velocity and gradient), a perturbation cut from one seeded
+    smoothed-noise volume at a seeded offset and scaled per shot, the source
+    at a seeded position in the window's central half and a receiver patch
+    centred on the source."""
+    H = host()
+    big_rng = np.random.default_rng(seed)
+    big = smoothed_noise(big_rng, (nz, nxy[1] + 64, nxy[1] + 64))
+    w = ricker(nw, dt, fpeak, delay)
+    wdev = np.zeros(nw, np.float32)
+    H.born_host_wavelet_deriv(nw, np.float32(dt), w, wdev)
+    tab = np.zeros((1, 8), np.float32)
+    H.born_host_sinc_table(1, 8, tab)
+    shots = []
+    for k in range(nshots):
+        rng = np.random.default_rng(seed + k)
+        mx, my = (int(v) for v in rng.integers(nxy[0], nxy[1] + 1, size=2))
+        mz = nz
+        s = Shot()
+        s.model_shape = (mz, my, mx)
+        pad = nbound + 4
+        s.nx, s.ny, s.nz = (H.born_host_padded_size(n, nbound, 4) for n in (mx, my, mz))
+        s.o = -pad * d
+        zm = np.arange(mz, dtype=np.float32) * np.float32(d)
+        v0 = np.float32(1500 + rng.uniform(-50, 50))
+        gz = np.float32(15 * rng.uniform(0.9, 1.1))
+        vz = np.clip(v0 + gz * np.sqrt(zm), 1400, 3000).astype(np.float32)
+        vel = np.ascontiguousarray(np.broadcast_to(vz[:, None, None], (mz, my, mx)), np.float32)
+        s.V = np.zeros(s.nx * s.ny * s.nz, np.float32)
+        H.born_host_rand_subspace(0, 0, 0, 0, np.float32(dt), mx, my, mz, 0.0, 0.0, 0.0, d, d,
+                                  d, vel.ravel(), s.nx, s.ny, s.nz, s.o, s.o, s.o, d, d, d,
+                                  s.V)
+        oy, ox = (int(v) for v in rng.integers(0, 64, size=2))
+        dv = np.ascontiguousarray(big[:, oy:oy + my, ox:ox + mx] * np.float32(rng.uniform(0.5, 1.5)))
+        img = np.zeros(s.nx * s.ny * s.nz, np.float32)
+        H.born_host_extract_sub(mx, my, mz, 0.0, 0.0, 0.0, d, d, d, dv.ravel(), s.nx, s.ny,
+                                s.nz, s.o, s.o, s.o, img)
+        s.model = np.zeros_like(img)
+        H.born_host_laplac_forward(s.nx, s.ny, s.nz, img, s.model)
+        sx = float(rng.uniform(0.25, 0.75) * (mx - 1) * d)
+        sy = float(rng.uniform(0.25, 0.75) * (my - 1) * d)
+        s.nt_big = nt + 8
+        s.src_locs = np.zeros(9, np.int32)
+        vals = np.zeros(9 * s.nt_big, np.float32)
+        H.born_host_source_func(s.nx, s.ny, s.o, s.o, s.o, d, d, d, sx, sy, src_depth, wdev,
+                                nw, s.nt_big, s.src_locs, vals)
+        s.src_vals = vals.reshape(9, s.nt_big)
+        s.rec_nx, s.rec_ny = rec
+        s.nrec = rec[0] * rec[1]
+        s.rec_locs = np.zeros(s.nrec, np.int32)
+        H.born_host_receiver_locs(rec[0], rec[1], sx - rec_d * (rec[0] - 1) / 2,
+                                  sy - rec_d * (rec[1] - 1) / 2, rec_d, rec_d, rec_depth,
+                                  s.nx, s.ny, s.o, s.o, s.o, d, d, d, s.rec_locs)
+        s.interior_cells = (s.nx - 8) * (s.ny - 8) * (s.nz - 8)
+        shots.append(s)
+    return shots, tab
+
+
 # Benchmark configurations (sizes of the model before padding)
 CONFIGS = {
     "c1": dict(n=(100, 100, 100), nt=800, nshots=1, rec=(50, 50), rec_d=20.0,
                rec_o=10.0, seed=1001, src_xy=[(495.0, 495.0)]),
     "c2": dict(n=(256, 256, 256), nt=1500, nshots=3, rec=(128, 128), rec_d=20.0,
                rec_o=10.0, seed=1002),
+}
+
+# Batched shots: 32 windows of [200, 320]^2 x 300 cells at 10 m, 2000 steps
+BATCHES = {
+    "c4": dict(nshots=32, nxy=(200, 320), nz=300, nt=2000, rec=(96, 96), seed=1004),
 }

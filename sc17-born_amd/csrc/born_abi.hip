@@ -197,18 +197,20 @@ born::Tune tuning(const born_ctx *c, int kind) {
 }
 
 // Autotuner: time every tile shape x z chunk for one step kernel and keep
-// the fastest.  run(tune) must launch the kernel on scratch buffers only.
+// the fastest in *tune / *tune_ms.  run(tune) must launch the kernel on
+// scratch buffers only.  Pair shapes (2 cells per lane) need even_nx.
 template <class F>
-int autotune(born_ctx *ctx, int kind, F run) {
-  if (!ctx->autotune || ctx->tune_ms[kind] >= 0.f) return BORN_OK;
+int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx, F run) {
+  if (!ctx->autotune || *tune_ms >= 0.f) return BORN_OK;
   static const int zcs[3] = {32, 64, 128};
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   float best = 1e30f;
-  born::Tune bt = ctx->tune[kind];
+  born::Tune bt = *tune;
   for (int sh = 0; sh < born::kNumShapes; sh++)
     for (int zi = 0; zi < 3; zi++) {
+      if (born::kShapes[sh].vx == 2 && !even_nx) continue;
       const born::Tune t{sh, ctx->zchunk > 0 ? ctx->zchunk : zcs[zi]};
       if (ctx->zchunk > 0 && zi > 0) continue;
       HIPCHK(run(t));  // warm
@@ -225,9 +227,14 @@ int autotune(born_ctx *ctx, int kind, F run) {
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  ctx->tune[kind] = bt;
-  ctx->tune_ms[kind] = best / 3.f;
+  *tune = bt;
+  *tune_ms = best / 3.f;
   return BORN_OK;
+}
+
+template <class F>
+int autotune(born_ctx *ctx, int kind, F run) {
+  return autotune_into(ctx, &ctx->tune[kind], &ctx->tune_ms[kind], ctx->nx % 2 == 0, run);
 }
 
 // Sponge factor per axis coordinate (cpu_prop.cc:141-165, 329-332).
@@ -438,6 +445,75 @@ int field_buf(born_ctx *ctx, float **p) {
   return BORN_OK;
 }
 
+// Receiver lookup of a domain: every plane holding receivers gets a slot;
+// rmap[slot][y][x] is the first receiver at that cell (or -1) and rnext[]
+// chains the receivers sharing a cell.
+struct RecTables {
+  std::vector<int> slot_of_z, rmap, rnext;
+  int nslots = 0, per_cell = 0;
+};
+
+bool interior_of(int nx, int ny, int nz, long long loc, int *x, int *y, int *z) {
+  const long long n12 = (long long)nx * ny;
+  if (loc < 0 || loc >= n12 * nz) return false;
+  *z = (int)(loc / n12);
+  *y = (int)((loc % n12) / nx);
+  *x = (int)(loc % nx);
+  return *x >= 4 && *x < nx - 4 && *y >= 4 && *y < ny - 4 && *z >= 4 && *z < nz - 4;
+}
+
+int receiver_tables(born_ctx *ctx, int nx, int ny, int nz, int nrec, const int *locs,
+                    RecTables *rt) {
+  std::vector<int> px(nrec), py(nrec), pz(nrec);
+  rt->slot_of_z.assign(nz, -1);
+  rt->nslots = 0;
+  for (int i = 0; i < nrec; i++) {
+    if (!interior_of(nx, ny, nz, locs[i], &px[i], &py[i], &pz[i]))
+      return fail(ctx, BORN_ERR_UNSUPPORTED,
+                  "receiver " + std::to_string(i) + " outside the stencil interior");
+    if (rt->slot_of_z[pz[i]] < 0) rt->slot_of_z[pz[i]] = rt->nslots++;
+  }
+  const long long n12 = (long long)nx * ny;
+  rt->rmap.assign((size_t)rt->nslots * n12, -1);
+  rt->rnext.assign(nrec, -1);
+  std::vector<int> last((size_t)rt->nslots * n12, -1), count((size_t)rt->nslots * n12, 0);
+  rt->per_cell = 0;
+  for (int i = 0; i < nrec; i++) {
+    const long long c = rt->slot_of_z[pz[i]] * n12 + (long long)py[i] * nx + px[i];
+    if (rt->rmap[c] < 0)
+      rt->rmap[c] = i;
+    else
+      rt->rnext[last[c]] = i;
+    last[c] = i;
+    rt->per_cell = std::max(rt->per_cell, ++count[c]);
+  }
+  return BORN_OK;
+}
+
+// cpu_prop.cc:296-326: the 13 stencil coefficients, double expressions
+// rounded to float
+void stencil_coeffs(float d1, float d2, float d3, float *c) {
+  auto C00 = [](double d) { return 8.0 / (5.0 * d * d); };
+  c[0] = (float)(-1025.0 / 576.0 * (C00(d1) + C00(d2) + C00(d3)));
+  const double dd[3] = {d1, d2, d3};
+  const double f[4] = {1.0, -8.0, 63.0, -896.0};
+  for (int k = 0; k < 4; k++)
+    for (int a = 0; a < 3; a++) c[1 + 3 * k + a] = (float)(k == 0 ? C00(dd[a]) : C00(dd[a]) / f[k]);
+}
+
+born::Taps taps_of(const float *coeffs) {
+  born::Taps t;
+  // reference tap indices: C0=0, CZk=1+3(k-1), CXk=2+3(k-1), CYk=3+3(k-1)
+  // (base/cpu_prop.cc:6-18)
+  t.c0 = coeffs[0];
+  for (int k = 0; k < 4; k++) {
+    t.cz[k] = coeffs[1 + 3 * k];
+    t.cx[k] = coeffs[2 + 3 * k];
+    t.cy[k] = coeffs[3 + 3 * k];
+  }
+  return t;
+}
+
 int check_jt(born_ctx *ctx, int jt) {
   // the forward indexes both sinc tables with the row it - (it/jt)*jt
   // (cpu_prop.cc:31-41)
@@ -517,14 +593,7 @@ int born_create_space(born_ctx *ctx, float d1, float d2, float d3, float bc_a,
   ctx->ny = ny;
   ctx->nz = nz;
   ctx->n = (long long)nx * ny * nz;
-  // cpu_prop.cc:296-326, double expressions rounded to float
-  auto C00 = [](double d) { return 8.0 / (5.0 * d * d); };
-  ctx->coeffs[0] = (float)(-1025.0 / 576.0 * (C00(d1) + C00(d2) + C00(d3)));
-  const double dd[3] = {d1, d2, d3};
-  const double f[4] = {1.0, -8.0, 63.0, -896.0};
-  for (int k = 0; k < 4; k++)
-    for (int a = 0; a < 3; a++)
-      ctx->coeffs[1 + 3 * k + a] = (float)(k == 0 ? C00(dd[a]) : C00(dd[a]) / f[k]);
+  stencil_coeffs(d1, d2, d3, ctx->coeffs);
   ctx->nbound = 40;
   ctx->bcB = .0005f;
   if (resized) {
@@ -658,45 +727,25 @@ int born_transfer_receiver_func(born_ctx *ctx, int rec_nx, int rec_ny, int nt_bi
   if (rec_nx < 1 || rec_ny < 1 || nt_big < 1 || !locs)
     return fail(ctx, BORN_ERR_ARG, "bad receiver geometry");
   const int nrec = rec_nx * rec_ny;
-  std::vector<int> px(nrec), py(nrec), pz(nrec);
-  std::vector<int> slot_of_z(ctx->nz, -1);
-  int nslots = 0;
-  for (int i = 0; i < nrec; i++) {
-    if (!interior(ctx, locs[i], &px[i], &py[i], &pz[i]))
-      return fail(ctx, BORN_ERR_UNSUPPORTED,
-                  "receiver " + std::to_string(i) + " outside the stencil interior");
-    if (slot_of_z[pz[i]] < 0) slot_of_z[pz[i]] = nslots++;
-  }
-  const long long n12 = (long long)ctx->nx * ctx->ny;
-  std::vector<int> rmap((size_t)nslots * n12, -1), rnext(nrec, -1), last((size_t)nslots * n12, -1);
-  std::vector<int> count((size_t)nslots * n12, 0);
-  int per_cell = 0;
-  for (int i = 0; i < nrec; i++) {
-    const long long c = slot_of_z[pz[i]] * n12 + (long long)py[i] * ctx->nx + px[i];
-    if (rmap[c] < 0)
-      rmap[c] = i;
-    else
-      rnext[last[c]] = i;
-    last[c] = i;
-    per_cell = std::max(per_cell, ++count[c]);
-  }
+  RecTables rt;
+  CHK(receiver_tables(ctx, ctx->nx, ctx->ny, ctx->nz, nrec, locs, &rt));
   CHK(set_device(ctx));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   CHK(dev_alloc(ctx, &ctx->d_plane_slot, (size_t)ctx->nz));
-  CHK(dev_alloc(ctx, &ctx->d_rmap, rmap.size()));
-  CHK(dev_alloc(ctx, &ctx->d_rnext, rnext.size()));
-  HIPCHK(hipMemcpy(ctx->d_plane_slot, slot_of_z.data(), ctx->nz * sizeof(int),
+  CHK(dev_alloc(ctx, &ctx->d_rmap, rt.rmap.size()));
+  CHK(dev_alloc(ctx, &ctx->d_rnext, rt.rnext.size()));
+  HIPCHK(hipMemcpy(ctx->d_plane_slot, rt.slot_of_z.data(), ctx->nz * sizeof(int),
                    hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->d_rmap, rmap.data(), rmap.size() * sizeof(int),
+  HIPCHK(hipMemcpy(ctx->d_rmap, rt.rmap.data(), rt.rmap.size() * sizeof(int),
                    hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->d_rnext, rnext.data(), rnext.size() * sizeof(int),
+  HIPCHK(hipMemcpy(ctx->d_rnext, rt.rnext.data(), rt.rnext.size() * sizeof(int),
                    hipMemcpyHostToDevice));
   ctx->rec_nx = rec_nx;
   ctx->rec_ny = rec_ny;
   ctx->nrec = nrec;
   ctx->nt_rec = nt_big;
-  ctx->nslots = nslots;
-  ctx->rec_per_cell = per_cell;
+  ctx->nslots = rt.nslots;
+  ctx->rec_per_cell = rt.per_cell;
   ctx->h_rec = traces;
   return BORN_OK;
 }
@@ -1203,12 +1252,381 @@ int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
   return BORN_OK;
 }
 
+int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y) {
+  if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
+  const born::TileShape &ts = born::kShapes[tuning(ctx, kernel).shape];
+  if (tile_x) *tile_x = born::kTX * ts.vx;
+  if (tile_y) *tile_y = ts.ry * ts.wy;
+  return BORN_OK;
+}
+
+int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_per_block,
+                    int cells_per_lane, int zchunk) {
+  if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
+  if (zchunk < 1) return fail(ctx, BORN_ERR_ARG, "zchunk must be positive");
+  for (int sh = 0; sh < born::kNumShapes; sh++) {
+    const born::TileShape &ts = born::kShapes[sh];
+    if (ts.ry == rows_per_thread && ts.wy == waves_per_block && ts.vx == cells_per_lane) {
+      ctx->tune[kernel] = {sh, zchunk};
+      ctx->tune_ms[kernel] = 0.f;  // fixed: the autotuner leaves it
+      return BORN_OK;
+    }
+  }
+  return fail(ctx, BORN_ERR_ARG, "no such launch shape");
+}
+
 void *born_get_stream(const born_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 int born_synchronize(born_ctx *ctx) {
   if (!ctx) return BORN_ERR_ARG;
   CHK(set_device(ctx));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BORN_OK;
+}
+
+// ============================================================ shot batches
+// Many shots with their own windowed domains advance together: one launch
+// per time step covers the tiles of every shot (march_batch_kernel), and
+// the time loop is captured once in a HIP graph and replayed.
+
+}  // extern "C"
+
+struct born_batch {
+  born_ctx *ctx = nullptr;
+  int nshots = 0;
+  std::vector<born::ShotDesc> desc;
+  born::ShotDesc *d_desc = nullptr;
+  float coeffs[13] = {};
+  float tapsum = 0.f;
+  long long cells = 0;
+  int ntb = 0;
+  bool even_nx = true;
+  std::vector<long long> src0;  // source samples of shot s in d_src ([npts][ntb])
+  int npts_total = 0;
+  float *d_f[4] = {nullptr, nullptr, nullptr, nullptr};
+  float *d_vel = nullptr, *d_model = nullptr, *d_sponge = nullptr, *d_src = nullptr;
+  float *d_amp = nullptr;
+  size_t amp_cap = 0;
+  float *d_tr = nullptr, *d_tmp = nullptr;
+  long long tr_total = 0;
+  int max_nrec = 0;
+  float *d_rings = nullptr;
+  int *d_pts = nullptr, *d_slot = nullptr, *d_rmap = nullptr, *d_rnext = nullptr;
+  born::Tune tune[4];
+  float tune_ms[4];
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int g_jt = -1, g_nt = -1, g_scatter = -1, g_storage = -1, g_shape = -1, g_zc = -1;
+};
+
+namespace {
+
+void batch_free(born_batch *b) {
+  born_ctx *ctx = b->ctx;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (int i = 0; i < 4; i++) (void)dev_alloc(ctx, &b->d_f[i], 0);
+  (void)dev_alloc(ctx, &b->d_vel, 0);
+  (void)dev_alloc(ctx, &b->d_model, 0);
+  (void)dev_alloc(ctx, &b->d_sponge, 0);
+  (void)dev_alloc(ctx, &b->d_src, 0);
+  (void)dev_alloc(ctx, &b->d_amp, 0);
+  (void)dev_alloc(ctx, &b->d_tr, 0);
+  (void)dev_alloc(ctx, &b->d_tmp, 0);
+  (void)dev_alloc(ctx, &b->d_rings, 0);
+  (void)dev_alloc(ctx, &b->d_pts, 0);
+  (void)dev_alloc(ctx, &b->d_slot, 0);
+  (void)dev_alloc(ctx, &b->d_rmap, 0);
+  (void)dev_alloc(ctx, &b->d_rnext, 0);
+  (void)dev_alloc(ctx, &b->d_desc, 0);
+}
+
+template <class T>
+int upload(born_ctx *ctx, T **dst, const std::vector<T> &v) {
+  CHK(dev_alloc(ctx, dst, std::max<size_t>(v.size(), 1)));
+  if (!v.empty())
+    HIPCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return BORN_OK;
+}
+
+int batch_build(born_batch *b, int nshots, const born_shot *shots) {
+  born_ctx *ctx = b->ctx;
+  b->nshots = nshots;
+  b->desc.assign(nshots, born::ShotDesc{});
+  b->src0.assign(nshots, 0);
+  b->ntb = shots[0].nt_big;
+  std::vector<float> sponge, src;
+  std::vector<int> pts, slots, rmap, rnext;
+  std::vector<float> bound(ctx->nbound);
+  for (int i = 0; i < ctx->nbound; i++)
+    bound[i] = expf(-ctx->bcB * ((float)ctx->nbound - i));
+  long long cells = 0, tr = 0;
+  for (int s = 0; s < nshots; s++) {
+    const born_shot &sh = shots[s];
+    born::ShotDesc &d = b->desc[s];
+    const std::string tag = "shot " + std::to_string(s) + ": ";
+    if (sh.nx < 9 || sh.ny < 9 || sh.nz < 9)
+      return fail(ctx, BORN_ERR_ARG, tag + "each axis needs at least 9 samples");
+    const long long n = (long long)sh.nx * sh.ny * sh.nz;
+    if (n > (1ll << 30)) return fail(ctx, BORN_ERR_UNSUPPORTED, tag + "more than 2^30 cells");
+    if (!sh.vel || !sh.model) return fail(ctx, BORN_ERR_ARG, tag + "null velocity or model");
+    if (sh.nt_big != b->ntb)
+      return fail(ctx, BORN_ERR_ARG, tag + "all shots need the same trace length");
+    if (sh.npts < 0 || sh.npts > born::kMaxPts || (sh.npts && (!sh.src_locs || !sh.src_vals)))
+      return fail(ctx, BORN_ERR_UNSUPPORTED, tag + "bad source (at most 32 points)");
+    if (sh.rec_nx < 1 || sh.rec_ny < 1 || !sh.rec_locs)
+      return fail(ctx, BORN_ERR_ARG, tag + "bad receiver geometry");
+    if (sh.nx % 2) b->even_nx = false;
+    d.nx = sh.nx;
+    d.ny = sh.ny;
+    d.nz = sh.nz;
+    d.npts = sh.npts;
+    d.cell0 = cells;
+    cells += n + (n & 1);  // keep every shot's first cell at an even offset
+    d.sp0 = (int)sponge.size();
+    for (int ax = 0; ax < 3; ax++) {
+      const int na = ax == 0 ? sh.nx : ax == 1 ? sh.ny : sh.nz;
+      std::vector<float> t = axis_sponge(na, ctx->nbound, bound);
+      sponge.insert(sponge.end(), t.begin(), t.end());
+    }
+    d.pts0 = (int)(pts.size() / 4);
+    for (int i = 0; i < sh.npts; i++) {
+      int x, y, z;
+      if (!interior_of(sh.nx, sh.ny, sh.nz, sh.src_locs[i], &x, &y, &z))
+        return fail(ctx, BORN_ERR_UNSUPPORTED,
+                    tag + "source point " + std::to_string(i) + " outside the interior");
+      pts.insert(pts.end(), {x, y, z, 0});
+    }
+    b->src0[s] = (long long)src.size();
+    if (sh.npts) src.insert(src.end(), sh.src_vals, sh.src_vals + (size_t)sh.npts * b->ntb);
+    const int nrec = sh.rec_nx * sh.rec_ny;
+    RecTables rt;
+    CHK(receiver_tables(ctx, sh.nx, sh.ny, sh.nz, nrec, sh.rec_locs, &rt));
+    d.slot0 = (int)slots.size();
+    slots.insert(slots.end(), rt.slot_of_z.begin(), rt.slot_of_z.end());
+    d.rmap0 = (long long)rmap.size();
+    rmap.insert(rmap.end(), rt.rmap.begin(), rt.rmap.end());
+    d.rnext0 = (int)rnext.size();
+    rnext.insert(rnext.end(), rt.rnext.begin(), rt.rnext.end());
+    d.nrec = nrec;
+    b->max_nrec = std::max(b->max_nrec, nrec);
+    d.tr0 = tr;
+    tr += (long long)nrec * b->ntb;
+    d.ring0 = s * born::kRingWords;
+    float vmax = 0.f;
+    for (long long i = 0; i < n; i++) vmax = std::max(vmax, fabsf(sh.vel[i]));
+    d.gain = 2.f + vmax * b->tapsum;
+  }
+  b->cells = cells;
+  b->tr_total = tr;
+  b->npts_total = (int)(pts.size() / 4);
+  CHK(set_device(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &b->d_f[i], (size_t)cells));
+  CHK(dev_alloc(ctx, &b->d_vel, (size_t)cells));
+  CHK(dev_alloc(ctx, &b->d_model, (size_t)cells));
+  for (int s = 0; s < nshots; s++) {
+    const born::ShotDesc &d = b->desc[s];
+    const size_t bytes = (size_t)d.nx * d.ny * d.nz * sizeof(float);
+    HIPCHK(hipMemcpy(b->d_vel + d.cell0, shots[s].vel, bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_model + d.cell0, shots[s].model, bytes, hipMemcpyHostToDevice));
+  }
+  CHK(upload(ctx, &b->d_sponge, sponge));
+  CHK(upload(ctx, &b->d_src, src));
+  CHK(upload(ctx, &b->d_pts, pts));
+  CHK(upload(ctx, &b->d_slot, slots));
+  CHK(upload(ctx, &b->d_rmap, rmap));
+  CHK(upload(ctx, &b->d_rnext, rnext));
+  CHK(upload(ctx, &b->d_desc, b->desc));
+  CHK(dev_alloc(ctx, &b->d_tr, (size_t)std::max(tr, 1ll)));
+  CHK(dev_alloc(ctx, &b->d_tmp, (size_t)b->max_nrec * b->ntb));
+  CHK(dev_alloc(ctx, &b->d_rings, (size_t)nshots * born::kRingWords));
+  for (int k = 0; k < 4; k++) {
+    b->tune[k] = {3, 64};
+    b->tune_ms[k] = -1.f;
+  }
+  return BORN_OK;
+}
+
+// the step launch of the batch at time step it
+born::BatchFwdStep batch_step(born_batch *b, int it, int jt, int scatter, bool half) {
+  born_ctx *ctx = b->ctx;
+  born::BatchFwdStep st{};
+  born::FwdStep &a = st.a;
+  const bool odd = it % 2;
+  a.s1 = b->d_f[odd ? 0 : 1];
+  a.s0 = b->d_f[odd ? 1 : 0];
+  a.r1 = b->d_f[odd ? 2 : 3];
+  a.r0 = b->d_f[odd ? 3 : 2];
+  a.vel = b->d_vel;
+  a.model = b->d_model;
+  a.src = {b->d_pts, 0};
+  a.amp = b->d_amp;
+  a.rec.plane_slot = b->d_slot;
+  a.rec.rmap = b->d_rmap;
+  a.rec.rnext = b->d_rnext;
+  a.rec.nrec = 0;
+  a.rec.ntb = b->ntb;
+  a.tr = b->d_tr;
+  const int id = it / jt, ii = it - id * jt;
+  a.tab = ctx->d_tab_d + (long long)ii * 8;
+  a.id = id;
+  a.extract = id + 7 < b->ntb;
+  a.scatter = scatter;
+  a.h.on = half;
+  a.h.sl = slots(it);
+  a.h.tapsum = b->tapsum;
+  st.sponge = b->d_sponge;
+  st.rings = b->d_rings;
+  st.desc = b->d_desc;
+  st.host_desc = b->desc.data();
+  st.nshots = b->nshots;
+  st.step = it;
+  st.npts_total = b->npts_total;
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int born_batch_create(born_ctx *ctx, int nshots, const born_shot *shots, float d1, float d2,
+                      float d3, born_batch **out) {
+  if (!ctx || !out) return BORN_ERR_ARG;
+  *out = nullptr;
+  if (nshots < 1 || nshots > born::kMaxBatch || !shots)
+    return fail(ctx, BORN_ERR_ARG,
+                "a batch holds 1 to " + std::to_string(born::kMaxBatch) + " shots");
+  if (!(d1 > 0) || !(d2 > 0) || !(d3 > 0))
+    return fail(ctx, BORN_ERR_ARG, "sampling must be positive");
+  born_batch *b = new born_batch;
+  b->ctx = ctx;
+  stencil_coeffs(d1, d2, d3, b->coeffs);
+  b->tapsum = fabsf(b->coeffs[0]);
+  for (int i = 1; i < 13; i++) b->tapsum += 2.f * fabsf(b->coeffs[i]);
+  const int rc = batch_build(b, nshots, shots);
+  if (rc != BORN_OK) {
+    batch_free(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return BORN_OK;
+}
+
+int born_batch_destroy(born_batch *b) {
+  if (!b) return BORN_ERR_ARG;
+  (void)set_device(b->ctx);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch_free(b);
+  delete b;
+  return BORN_OK;
+}
+
+int born_batch_forward(born_batch *b, int jt, int nt, int scatter) {
+  if (!b) return BORN_ERR_ARG;
+  born_ctx *ctx = b->ctx;
+  if (!ctx->d_tab_s || !ctx->d_tab_d)
+    return fail(ctx, BORN_ERR_STATE, "sinc tables have not been transferred");
+  CHK(check_jt(ctx, jt));
+  if (nt < 0 || (scatter != 0 && scatter != 1))
+    return fail(ctx, BORN_ERR_ARG, "bad forward arguments");
+  CHK(set_device(ctx));
+  const bool half = ctx->storage == BORN_STORAGE_F16;
+  const int kind = (half ? 2 : 0) + scatter;
+  const born::Taps taps = taps_of(b->coeffs);
+  CHK(ensure(ctx, &b->d_amp, &b->amp_cap,
+             (size_t)std::max(1, b->npts_total) * std::max(nt, 1)));
+  {  // tune on the batch arrays (cleared below), no traces written
+    HIPCHK(born::launch_batch_ring_init(b->d_rings, b->nshots, ctx->stream));
+    born::BatchFwdStep t = batch_step(b, 0, jt, scatter, half);
+    t.a.extract = 0;
+    CHK(autotune_into(ctx, &b->tune[kind], &b->tune_ms[kind], b->even_nx,
+                      [&](const born::Tune &tu) {
+                        return born::launch_batch_fwd_step(taps, tu, t, ctx->stream);
+                      }));
+  }
+  born::Tune tu = b->tune[kind];
+  if (ctx->zchunk > 0) tu.zchunk = ctx->zchunk;
+  if (born::kShapes[tu.shape].vx == 2 && !b->even_nx) tu.shape = 3;
+  // initial state: fields and traces zero, amplitudes of every shot, scales
+  for (int i = 0; i < 4; i++) CHK(zero(ctx, b->d_f[i], b->cells));
+  CHK(zero(ctx, b->d_tr, b->tr_total));
+  for (int s = 0; s < b->nshots; s++) {
+    const born::ShotDesc &d = b->desc[s];
+    if (d.npts)
+      HIPCHK(born::launch_src_amp(b->d_amp + d.pts0, nt, d.npts, b->d_src + b->src0[s], b->ntb,
+                                  ctx->d_tab_s, ctx->nsinc_s, jt, ctx->jts, 0, 1 / (float)jt,
+                                  ctx->stream, b->npts_total));
+  }
+  if (half) {
+    HIPCHK(born::launch_batch_ring_init(b->d_rings, b->nshots, ctx->stream));
+    for (int s = 0; s < b->nshots; s++) {
+      const born::ShotDesc &d = b->desc[s];
+      HIPCHK(born::launch_absmax(b->d_model + d.cell0, (long long)d.nx * d.ny * d.nz,
+                                 (unsigned *)(b->d_rings + d.ring0 + 16), ctx->stream));
+    }
+  }
+  if (nt == 0) return BORN_OK;
+  // the time loop: captured once per (jt, nt, scatter, storage, shape), replayed
+  const bool same = b->exec && b->g_jt == jt && b->g_nt == nt && b->g_scatter == scatter &&
+                    b->g_storage == ctx->storage && b->g_shape == tu.shape &&
+                    b->g_zc == tu.zchunk;
+  if (!same) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t le = hipSuccess;
+    for (int it = 0; it < nt && le == hipSuccess; it++)
+      le = born::launch_batch_fwd_step(taps, tu, batch_step(b, it, jt, scatter, half),
+                                       ctx->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(ctx, BORN_ERR_HIP,
+                  std::string("capture of the batch time loop: ") +
+                      hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->g_jt = jt;
+    b->g_nt = nt;
+    b->g_scatter = scatter;
+    b->g_storage = ctx->storage;
+    b->g_shape = tu.shape;
+    b->g_zc = tu.zchunk;
+  }
+  HIPCHK(hipGraphLaunch(b->exec, ctx->stream));
+  return BORN_OK;
+}
+
+int born_batch_traces(born_batch *b, int shot, float *rec) {
+  if (!b || !rec) return BORN_ERR_ARG;
+  born_ctx *ctx = b->ctx;
+  if (shot < 0 || shot >= b->nshots) return fail(ctx, BORN_ERR_ARG, "no such shot");
+  CHK(set_device(ctx));
+  const born::ShotDesc &d = b->desc[shot];
+  HIPCHK(born::launch_transpose(b->d_tmp, b->d_tr + d.tr0, b->ntb, d.nrec, ctx->stream));
+  HIPCHK(hipMemcpyAsync(rec, b->d_tmp, (size_t)d.nrec * b->ntb * sizeof(float),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BORN_OK;
+}
+
+int born_batch_get_tile(const born_batch *b, int scatter, int *tile_x, int *tile_y,
+                        int *zchunk, float *ms) {
+  if (!b || (scatter != 0 && scatter != 1)) return BORN_ERR_ARG;
+  const int kind = (b->ctx->storage == BORN_STORAGE_F16 ? 2 : 0) + scatter;
+  const born::Tune t = b->tune[kind];
+  if (tile_x) *tile_x = born::kTX * born::kShapes[t.shape].vx;
+  if (tile_y) *tile_y = born::kShapes[t.shape].ry * born::kShapes[t.shape].wy;
+  if (zchunk) *zchunk = t.zchunk;
+  if (ms) *ms = b->tune_ms[kind];
   return BORN_OK;
 }
 

@@ -118,6 +118,24 @@ __device__ __forceinline__ float ring_max(const Ring &r, int s) {
   return __uint_as_float(r.maxb[s]);
 }
 
+// Read-only tables indexed by a uniform value (sponge factor of a plane, the
+// receiver slot of a plane, source points): read through the constant
+// address space, so they become scalar loads, tracked apart from the vector
+// loads of the wavefields.  A vector load issued after the next plane's
+// prefetch would otherwise make the wait for it a wait for the prefetch too.
+template <class T>
+__device__ __forceinline__ T ldc(const T *p, long long i) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit words");
+  typedef const __attribute__((address_space(4))) uint32_t *cwords;
+  const cwords w = (cwords)(p + i);
+  uint32_t u[sizeof(T) / 4];
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); k++) u[k] = w[k];
+  T v;
+  __builtin_memcpy(&v, u, sizeof(T));
+  return v;
+}
+
 // The hardware hands block h to XCD h % 8.  Give every XCD a contiguous range
 // of logical tiles so that tiles sharing halo rows share an L2.
 __device__ __forceinline__ int xcd_remap(int h, int nb) {
@@ -142,7 +160,7 @@ __device__ __forceinline__ uint32_t points_in_block(const Points &pts, int x0,
                                                    int z0, int z1) {
   uint32_t m = 0;
   for (int i = 0; i < pts.n; i++) {
-    const int4 q = pts.p[i];
+    const int4 q = ldc(pts.p, i);
     if (q.x >= x0 && q.x < x1 && q.y >= y0 && q.y < y1 && q.z >= z0 && q.z < z1)
       m |= 1u << i;
   }
@@ -154,7 +172,7 @@ __device__ __forceinline__ uint32_t points_on_plane(const Points &pts,
   uint32_t m = 0;
   for (uint32_t b = mask; b; b &= b - 1) {
     const int i = __builtin_ctz(b);
-    if (pts.p[i].z == z) m |= 1u << i;
+    if (ldc(pts.p, i).z == z) m |= 1u << i;
   }
   return m;
 }
@@ -164,8 +182,8 @@ __device__ __forceinline__ float add_points(const Points &pts, uint32_t m, int x
                                             int y, float v, const float *amp) {
   for (uint32_t b = m; b; b &= b - 1) {
     const int i = __builtin_ctz(b);
-    const int4 q = pts.p[i];
-    if (q.x == x && q.y == y) v = __fadd_rn(v, amp[i]);
+    const int4 q = ldc(pts.p, i);
+    if (q.x == x && q.y == y) v = __fadd_rn(v, ldc(amp, i));
   }
   return v;
 }
@@ -176,26 +194,72 @@ struct CellArgs {
   float b;     // sponge factor (1 outside the sponge)
 };
 
-// Generic z-marching leapfrog kernel.  Op supplies (all const):
-//   static constexpr int NF, NPRE, NMAX; static constexpr bool DAMP;
-//   Blk block(x0, x1, y0, y1, z0, z1);       uniform per-block state
-//   float ld(const Blk&, int f, uint32_t o);  current level of field f
-//   Pln plane(const Blk&, int z);            uniform per-plane state
-//   void prefetch(const Blk&, float (&pre)[NPRE], uint32_t o);
+// ---- VX consecutive cells of a row (VX = 1 or 2).  A pair starts at an
+// even element offset (x tiles start at even x and nx is even when VX = 2),
+// so float2 / __half2 accesses are naturally aligned.
+template <int VX>
+__device__ __forceinline__ void ld_vec(const float *p, uint32_t o, float, float (&v)[VX]) {
+  if constexpr (VX == 1) {
+    v[0] = at(p, o);
+  } else {
+    const float2 w = at((const float2 *)p, o >> 1);
+    v[0] = w.x;
+    v[1] = w.y;
+  }
+}
+template <int VX>
+__device__ __forceinline__ void ld_vec(const __half *p, uint32_t o, float inv,
+                                       float (&v)[VX]) {
+  if constexpr (VX == 1) {
+    v[0] = __half2float(at(p, o)) * inv;
+  } else {
+    const __half2 w = at((const __half2 *)p, o >> 1);
+    v[0] = __low2float(w) * inv;
+    v[1] = __high2float(w) * inv;
+  }
+}
+template <int VX>
+__device__ __forceinline__ void st_vec(float *p, uint32_t o, const float (&v)[VX], float) {
+  if constexpr (VX == 1)
+    at(p, o) = v[0];
+  else
+    at((float2 *)p, o >> 1) = make_float2(v[0], v[1]);
+}
+__device__ __forceinline__ float sat_half(float v) { return fminf(fmaxf(v, -65504.f), 65504.f); }
+template <int VX>
+__device__ __forceinline__ void st_vec(__half *p, uint32_t o, const float (&v)[VX],
+                                       float scale) {
+  if constexpr (VX == 1)
+    at(p, o) = __float2half_rn(sat_half(v[0] * scale));
+  else
+    at((__half2 *)p, o >> 1) =
+        __floats2half2_rn(sat_half(v[0] * scale), sat_half(v[1] * scale));
+}
+
+// Generic z-marching leapfrog tile.  VX cells per lane along x (a tile is
+// 64*VX x RY*WY cells).  Op supplies (all const):
+//   static constexpr int NF, NPRE, NOUT, NMAX; static constexpr bool DAMP;
+//   Blk block(x0, x1, y0, y1, z0, z1);          uniform per-block state
+//   void first(const Blk&);                     (NMAX > 0) once per launch
+//   ldv<VX>(const Blk&, int f, uint32_t o, float (&v)[VX])   current level
+//   Pln plane(const Blk&, int z);               uniform per-plane state
+//   prefetchv<VX>(const Blk&, float (&p)[NPRE][VX], uint32_t o)
 //   void cell(const Blk&, const Pln&, const CellArgs&, const float *lap,
-//             const float *ctr, const float (&pre)[NPRE], float (&tmax)[NF]);
-//   void first(const Blk&);                  (NMAX > 0) once per launch
-//   void publish_max(int f, float m);        (NMAX > 0) block max of |new|
-template <class Op, int RY, int WY>
+//             const float *ctr, const float *pre, float (&out)[NOUT],
+//             float (&tmax)[NF]);                the new values of one cell
+//   storev<VX>(const Blk&, uint32_t o, const float (&out)[NOUT][VX])
+//   void publish_max(int f, float m);           (NMAX > 0) block max of |new|
+template <class Op, int RY, int WY, int VX>
 __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const Sponge &sp,
                                            const Op &op, const int L) {
   constexpr int NF = Op::NF;
   constexpr int NP = Op::NPRE > 0 ? Op::NPRE : 1;
+  constexpr int NO = Op::NOUT;
+  constexpr int TW = kTX * VX;
   constexpr int TY = RY * WY;
-  constexpr int LW = kTX + 2 * kH;
+  constexpr int LW = TW + 2 * kH;
   constexpr int LH = TY + 2 * kH;
-  constexpr int NXH = 2 * kH * TY;  // x-halo cells of a plane
-  __shared__ float tile[NF][LH][LW];
+  __shared__ __align__(16) float tile[NF][LH][LW];
 
   const int bx = L % g.ntx;
   const int rest = L / g.ntx;
@@ -205,14 +269,14 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   const int ty = (int)threadIdx.x / kTX;
   // x tiles start at the row start so that a wavefront's row maps onto whole
   // cache lines; the 4-cell shell at either end is masked
-  const int x0 = bx * kTX, y0 = kH + by * TY;
+  const int x0 = bx * TW, y0 = kH + by * TY;
   const int z0 = kH + bz * g.zc;
   const int z1 = min(z0 + g.zc, g.nz - kH);
-  const int x = x0 + tx;
-  const bool xin = x >= kH && x < g.nx - kH;
-  const int xc = min(x, g.nx - 1);
+  const int x = x0 + VX * tx;
+  const bool xin = x >= kH && x < g.nx - kH;  // with VX = 2 a pair is in or out
+  const int xc = min(x, g.nx - VX);
 
-  const auto blk = op.block(max(x0, kH), min(x0 + kTX, g.nx - kH), y0,
+  const auto blk = op.block(max(x0, kH), min(x0 + TW, g.nx - kH), y0,
                             min(y0 + TY, g.ny - kH), z0, z1);
   if constexpr (Op::NMAX > 0)
     if (L == 0 && threadIdx.x == 0) op.first(blk);
@@ -220,21 +284,24 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   int yy[RY];
   bool in[RY];
   uint32_t row[RY];
-  float bxy[RY];
+  float bxy[RY][VX];
 #pragma unroll
   for (int j = 0; j < RY; j++) {
     yy[j] = y0 + ty + WY * j;
     in[j] = xin && yy[j] < g.ny - kH;
     const int yc = min(yy[j], g.ny - 1);
     row[j] = (uint32_t)yc * g.nx + xc;
-    bxy[j] = Op::DAMP ? fminf(sp.bx[xc], sp.by[yc]) : 1.f;
+#pragma unroll
+    for (int v = 0; v < VX; v++) bxy[j][v] = Op::DAMP ? fminf(sp.bx[xc + v], sp.by[yc]) : 1.f;
   }
 
-  // LDS halo: 8 y-halo rows of 64 cells and 8 x-halo columns of TY rows,
-  // spread over the block's threads
-  constexpr int NYH = 2 * kH * kTX;          // y-halo cells of a plane
+  // LDS halo in units of VX cells: 8 y-halo rows of 64 units and, per row,
+  // kH/VX units on either side, spread over the block's threads
+  constexpr int XU = kH / VX;
+  constexpr int NYH = 2 * kH * kTX;
+  constexpr int NXH = 2 * XU * TY;
   constexpr int NHALO = NYH + NXH;
-  constexpr int HPT = (NHALO + kTX * WY - 1) / (kTX * WY);  // per thread
+  constexpr int HPT = (NHALO + kTX * WY - 1) / (kTX * WY);
   uint32_t hoff[HPT];
   short hr[HPT], hc[HPT];
   bool hv[HPT];
@@ -243,27 +310,37 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     const int h = (int)threadIdx.x + k * kTX * WY;
     hv[k] = h < NHALO;
     int lr, lc;
-    if (h < NYH) {  // rows 0..3 and TY+4..TY+7, columns 4..67
+    if (h < NYH) {  // rows 0..3 and TY+4..TY+7
       const int r = h / kTX;
       lr = r < kH ? r : TY + r;
-      lc = kH + h % kTX;
-    } else {  // columns 0..3 and 68..71 of rows 4..TY+3
+      lc = kH + VX * (h % kTX);
+    } else {  // columns 0..3 and TW+4..TW+7 of rows 4..TY+3
       const int hh = min(h - NYH, NXH - 1);
-      const int c = hh % (2 * kH);
-      lr = kH + hh / (2 * kH);
-      lc = c < kH ? c : kTX + c;
+      const int c = hh % (2 * XU);
+      lr = kH + hh / (2 * XU);
+      lc = c < XU ? VX * c : kH + TW + VX * (c - XU);
     }
     hr[k] = (short)lr;
     hc[k] = (short)lc;
     const int gy = min(max(y0 - kH + lr, 0), g.ny - 1);
-    const int gx = min(max(x0 - kH + lc, 0), g.nx - 1);
+    const int gx = min(max(x0 - kH + lc, 0), g.nx - VX);
     hoff[k] = (uint32_t)gy * g.nx + gx;
   }
 
-  float q[NF][RY][2 * kH + 1];
-  float lead[NF][RY];
-  float halo[NF][HPT];
-  float pre[RY][NP], pre_n[RY][NP];
+  // One plane of loads ahead: the next plane's queue lead, LDS halo and
+  // pointwise inputs are issued before this plane is computed.  (Stores and
+  // loads share one counter on this architecture, so with the plane's stores
+  // pending any wait for a load drains every load issued so far: a deeper
+  // register pipeline would not keep more loads in flight.)
+  // The LDS halo of a plane is loaded together with the plane's centre rows
+  // (when the plane enters the queue as its lead), not kH planes later: the
+  // neighbouring tiles read those rows at about the same time, so they come
+  // from L2; kH planes later the L2 has turned over and they would be
+  // fetched from HBM again.  hq[k] holds the halo of plane z + k.
+  float q[NF][RY][VX][2 * kH + 1];
+  float lead[NF][RY][VX];
+  float hq[kH + 1][NF][HPT][VX];
+  float pre[RY][NP][VX], pre_n[RY][NP][VX];
   float tmax[NF];
 #pragma unroll
   for (int f = 0; f < NF; f++) {
@@ -271,16 +348,45 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
 #pragma unroll
     for (int j = 0; j < RY; j++) {
 #pragma unroll
-      for (int k = 0; k < 2 * kH; k++)
-        q[f][j][k] = op.ld(blk, f, (uint32_t)(z0 - kH + k) * g.plane + row[j]);
-      lead[f][j] = op.ld(blk, f, (uint32_t)(z0 + kH) * g.plane + row[j]);
+      for (int k = 0; k < 2 * kH; k++) {
+        float w[VX];
+        op.template ldv<VX>(blk, f, (uint32_t)(z0 - kH + k) * g.plane + row[j], w);
+#pragma unroll
+        for (int v = 0; v < VX; v++) q[f][j][v][k] = w[v];
+      }
+      op.template ldv<VX>(blk, f, (uint32_t)(z0 + kH) * g.plane + row[j], lead[f][j]);
     }
 #pragma unroll
-    for (int k = 0; k < HPT; k++)
-      halo[f][k] = hv[k] ? op.ld(blk, f, (uint32_t)z0 * g.plane + hoff[k]) : 0.f;
+    for (int p = 0; p <= kH; p++) {
+      const int zp = min(z0 + p, g.nz - 1);
+#pragma unroll
+      for (int k = 0; k < HPT; k++) {
+        if (hv[k]) {
+          op.template ldv<VX>(blk, f, (uint32_t)zp * g.plane + hoff[k], hq[p][f][k]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < VX; v++) hq[p][f][k][v] = 0.f;
+        }
+      }
+    }
   }
 #pragma unroll
-  for (int j = 0; j < RY; j++) op.prefetch(blk, pre[j], (uint32_t)z0 * g.plane + row[j]);
+  for (int j = 0; j < RY; j++)
+    op.template prefetchv<VX>(blk, pre[j], (uint32_t)z0 * g.plane + row[j]);
+
+  // The new values of a plane are stored at the start of the next plane,
+  // after its wait and before its loads: the wait for a plane's inputs then
+  // finds stores and loads that were issued a whole plane earlier.
+  float pend[RY][NO][VX];
+  uint32_t pend_o = 0;
+  bool pend_on = false;
+  auto flush = [&]() {
+    if (pend_on) {
+#pragma unroll
+      for (int j = 0; j < RY; j++)
+        if (in[j]) op.template storev<VX>(blk, pend_o + row[j], pend[j]);
+    }
+  };
 
   for (int z = z0; z < z1; ++z) {
     const uint32_t zo = (uint32_t)z * g.plane;
@@ -288,70 +394,138 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     for (int f = 0; f < NF; f++) {
 #pragma unroll
       for (int j = 0; j < RY; j++) {
-        q[f][j][2 * kH] = lead[f][j];
-        tile[f][kH + ty + WY * j][kH + tx] = q[f][j][kH];
+#pragma unroll
+        for (int v = 0; v < VX; v++) q[f][j][v][2 * kH] = lead[f][j][v];
+        float *d = &tile[f][kH + ty + WY * j][kH + VX * tx];
+        if constexpr (VX == 1)
+          d[0] = q[f][j][0][kH];
+        else
+          *(float2 *)d = make_float2(q[f][j][0][kH], q[f][j][1][kH]);
       }
 #pragma unroll
-      for (int k = 0; k < HPT; k++)
-        if (hv[k]) tile[f][hr[k]][hc[k]] = halo[f][k];
+      for (int k = 0; k < HPT; k++) {
+        if (hv[k]) {
+          float *d = &tile[f][hr[k]][hc[k]];
+          if constexpr (VX == 1)
+            d[0] = hq[0][f][k][0];
+          else
+            *(float2 *)d = make_float2(hq[0][f][k][0], hq[0][f][k][1]);
+        }
+      }
     }
-    // next plane's loads, in flight during this plane's compute
+#pragma unroll
+    for (int p = 0; p < kH; p++)
+#pragma unroll
+      for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int k = 0; k < HPT; k++)
+#pragma unroll
+          for (int v = 0; v < VX; v++) hq[p][f][k][v] = hq[p + 1][f][k][v];
+    flush();
+    // next plane's loads, in flight during this plane's compute: the queue
+    // lead z+1+kH with its halo, and the pointwise inputs of z+1
     if (z + 1 < z1) {
       const uint32_t zn = zo + g.plane;
+      const uint32_t zh = (uint32_t)min(z + 1 + kH, g.nz - 1) * g.plane;
 #pragma unroll
       for (int f = 0; f < NF; f++) {
 #pragma unroll
-        for (int j = 0; j < RY; j++) lead[f][j] = op.ld(blk, f, zn + kH * g.plane + row[j]);
+        for (int j = 0; j < RY; j++)
+          op.template ldv<VX>(blk, f, zn + kH * g.plane + row[j], lead[f][j]);
 #pragma unroll
         for (int k = 0; k < HPT; k++)
-          if (hv[k]) halo[f][k] = op.ld(blk, f, zn + hoff[k]);
+          if (hv[k]) op.template ldv<VX>(blk, f, zh + hoff[k], hq[kH][f][k]);
       }
 #pragma unroll
-      for (int j = 0; j < RY; j++) op.prefetch(blk, pre_n[j], zn + row[j]);
+      for (int j = 0; j < RY; j++) op.template prefetchv<VX>(blk, pre_n[j], zn + row[j]);
     }
     __syncthreads();
     const auto pl = op.plane(blk, z);
-    const float bzv = Op::DAMP ? sp.bz[z] : 1.f;
+    const float bzv = Op::DAMP ? ldc(sp.bz, z) : 1.f;
 #pragma unroll
     for (int j = 0; j < RY; j++) {
       if (!in[j]) continue;
-      const int lr = kH + ty + WY * j, lc = kH + tx;
-      float lap[NF], ctr[NF];
+      const int lr = kH + ty + WY * j, lc = kH + VX * tx;
+      float lap[VX][NF], ctr[VX][NF];
 #pragma unroll
       for (int f = 0; f < NF; f++) {
-        float a = t.c0 * q[f][j][kH];
+        // the row from lc - 4 to lc + VX + 3, and the column pairs above/below
+        float xs[2 * kH + VX];
+        float up[kH][VX], dn[kH][VX];
+        if constexpr (VX == 1) {
 #pragma unroll
-        for (int k = 1; k <= kH; k++)
-          a += t.cx[k - 1] * (tile[f][lr][lc - k] + tile[f][lr][lc + k]);
+          for (int i = 0; i < 2 * kH + 1; i++) xs[i] = tile[f][lr][lc - kH + i];
 #pragma unroll
-        for (int k = 1; k <= kH; k++)
-          a += t.cy[k - 1] * (tile[f][lr - k][lc] + tile[f][lr + k][lc]);
+          for (int k = 1; k <= kH; k++) {
+            up[k - 1][0] = tile[f][lr - k][lc];
+            dn[k - 1][0] = tile[f][lr + k][lc];
+          }
+        } else {
 #pragma unroll
-        for (int k = 1; k <= kH; k++)
-          a += t.cz[k - 1] * (q[f][j][kH - k] + q[f][j][kH + k]);
-        lap[f] = a;
-        ctr[f] = q[f][j][kH];
+          for (int i = 0; i < kH + 1; i++) {
+            const float2 w = *(const float2 *)&tile[f][lr][lc - kH + 2 * i];
+            xs[2 * i] = w.x;
+            xs[2 * i + 1] = w.y;
+          }
+#pragma unroll
+          for (int k = 1; k <= kH; k++) {
+            const float2 a = *(const float2 *)&tile[f][lr - k][lc];
+            const float2 b = *(const float2 *)&tile[f][lr + k][lc];
+            up[k - 1][0] = a.x;
+            up[k - 1][1] = a.y;
+            dn[k - 1][0] = b.x;
+            dn[k - 1][1] = b.y;
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < VX; v++) {
+          float a = t.c0 * q[f][j][v][kH];
+#pragma unroll
+          for (int k = 1; k <= kH; k++) a += t.cx[k - 1] * (xs[kH + v - k] + xs[kH + v + k]);
+#pragma unroll
+          for (int k = 1; k <= kH; k++) a += t.cy[k - 1] * (up[k - 1][v] + dn[k - 1][v]);
+#pragma unroll
+          for (int k = 1; k <= kH; k++)
+            a += t.cz[k - 1] * (q[f][j][v][kH - k] + q[f][j][v][kH + k]);
+          lap[v][f] = a;
+          ctr[v][f] = q[f][j][v][kH];
+        }
       }
-      CellArgs c;
-      c.x = x;
-      c.y = yy[j];
-      c.z = z;
-      c.o = zo + row[j];
-      c.b = fminf(bxy[j], bzv);
-      op.cell(blk, pl, c, lap, ctr, pre[j], tmax);
+#pragma unroll
+      for (int v = 0; v < VX; v++) {
+        CellArgs c;
+        c.x = x + v;
+        c.y = yy[j];
+        c.z = z;
+        c.o = zo + row[j] + v;
+        c.b = fminf(bxy[j][v], bzv);
+        float p[NP], o[NO];
+#pragma unroll
+        for (int k = 0; k < NP; k++) p[k] = pre[j][k][v];
+        op.cell(blk, pl, c, lap[v], ctr[v], p, o, tmax);
+#pragma unroll
+        for (int k = 0; k < NO; k++) pend[j][k][v] = o[k];
+      }
     }
+    pend_o = zo;
+    pend_on = true;
     __syncthreads();
 #pragma unroll
     for (int f = 0; f < NF; f++)
 #pragma unroll
       for (int j = 0; j < RY; j++)
 #pragma unroll
-        for (int k = 0; k < 2 * kH; k++) q[f][j][k] = q[f][j][k + 1];
+        for (int v = 0; v < VX; v++)
+#pragma unroll
+          for (int k = 0; k < 2 * kH; k++) q[f][j][v][k] = q[f][j][v][k + 1];
 #pragma unroll
     for (int j = 0; j < RY; j++)
 #pragma unroll
-      for (int k = 0; k < NP; k++) pre[j][k] = pre_n[j][k];
+      for (int k = 0; k < NP; k++)
+#pragma unroll
+        for (int v = 0; v < VX; v++) pre[j][k][v] = pre_n[j][k][v];
   }
+  flush();
   // block maxima of |new level| for the next step's FP16 scales
   if constexpr (Op::NMAX > 0) {
     __shared__ float red[NF][WY];
@@ -373,10 +547,10 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   }
 }
 
-template <class Op, int RY, int WY>
+template <class Op, int RY, int WY, int VX>
 __global__ void __launch_bounds__(kTX *WY)
     march_kernel(const Grid g, const Taps t, const Sponge sp, const Op op) {
-  march_tile<Op, RY, WY>(g, t, sp, op, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+  march_tile<Op, RY, WY, VX>(g, t, sp, op, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // Tiles of a batch of shots with their own grids: shot s owns the logical
@@ -388,31 +562,31 @@ struct BatchTiles {
   int tile0[kMaxBatch + 1];
 };
 
-// Tile grid of a domain for tile shape (RY, WY) and z chunk zc.
-__host__ __device__ inline Grid tile_grid(int nx, int ny, int nz, int ty, int zc) {
+// Tile grid of a domain for tiles of tw x ty cells and z chunk zc.
+__host__ __device__ inline Grid tile_grid(int nx, int ny, int nz, int tw, int ty, int zc) {
   Grid g;
   g.nx = nx;
   g.ny = ny;
   g.nz = nz;
   g.plane = (uint32_t)nx * ny;
-  g.ntx = (nx - kH + kTX - 1) / kTX;
+  g.ntx = (nx - kH + tw - 1) / tw;
   g.nty = (ny - 2 * kH + ty - 1) / ty;
   g.zc = zc;
   g.ntz = (nz - 2 * kH + zc - 1) / zc;
   return g;
 }
 
-// One launch advances every shot of a batch by one step.  BOp::shot(s, ty,
-// zc) returns {Grid, Sponge, Op} of shot s with the op's pointers moved to
+// One launch advances every shot of a batch by one step.  BOp::shot(s, tw,
+// ty, zc) returns {Grid, Sponge, Op} of shot s with the op's pointers moved to
 // the shot's part of the batch arrays.
-template <class BOp, int RY, int WY>
+template <class BOp, int RY, int WY, int VX>
 __global__ void __launch_bounds__(kTX *WY)
     march_batch_kernel(const BatchTiles bt, const Taps t, const BOp bop) {
   const int Lg = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   int s = 0;
   while (s + 1 < bt.n && Lg >= bt.tile0[s + 1]) s++;
-  const auto sh = bop.shot(s, RY * WY, bt.zc);
-  march_tile<typename BOp::Op, RY, WY>(sh.g, t, sh.sp, sh.op, Lg - bt.tile0[s]);
+  const auto sh = bop.shot(s, kTX * VX, RY * WY, bt.zc);
+  march_tile<typename BOp::Op, RY, WY, VX>(sh.g, t, sh.sp, sh.op, Lg - bt.tile0[s]);
 }
 
 }  // namespace born

@@ -56,7 +56,7 @@ struct Blk2 {  // two stored fields
 
 __device__ __forceinline__ float amp_sum(const Points &pts, const float *amp) {
   float a = 0.f;
-  for (int i = 0; i < pts.n; i++) a += fabsf(amp[i]);
+  for (int i = 0; i < pts.n; i++) a += fabsf(ldc(amp, i));
   return a;
 }
 
@@ -64,7 +64,7 @@ __device__ __forceinline__ float amp_sum(const Points &pts, const float *amp) {
 template <int SCATTER, class T>
 struct FwdOp {
   static constexpr bool H = kHalf<T>;
-  static constexpr int NF = 2, NPRE = 4, NMAX = H ? 2 : 0;
+  static constexpr int NF = 2, NPRE = 4, NOUT = 3, NMAX = H ? 2 : 0;
   static constexpr bool DAMP = true;
   const T *__restrict__ s1;
   const T *__restrict__ r1;
@@ -117,21 +117,29 @@ struct FwdOp {
   __device__ void publish_max(int f, float m) const {
     atomicMax(&ring[f].maxb[sl.nxt], __float_as_uint(m));
   }
-  __device__ float ld(const Blk2 &b, int f, uint32_t o) const {
-    return ld_level(f == 0 ? s1 : r1, o, b.ic[f]);
+  template <int VX>
+  __device__ void ldv(const Blk2 &b, int f, uint32_t o, float (&v)[VX]) const {
+    ld_vec<VX>(f == 0 ? s1 : r1, o, b.ic[f], v);
   }
   __device__ SrcPln plane(const Blk2 &b, int z) const {
-    return {points_on_plane(pts, b.pm, z), rm.plane_slot[z]};
+    return {points_on_plane(pts, b.pm, z), ldc(rm.plane_slot, z)};
   }
-  __device__ void prefetch(const Blk2 &b, float (&p)[NPRE], uint32_t o) const {
-    p[0] = ld_level(s0, o, b.io[0]);
-    p[1] = ld_level(r0, o, b.io[1]);
-    p[2] = at(vel, o);
-    p[3] = at(model, o);
+  template <int VX>
+  __device__ void prefetchv(const Blk2 &b, float (&p)[NPRE][VX], uint32_t o) const {
+    ld_vec<VX>(s0, o, b.io[0], p[0]);
+    ld_vec<VX>(r0, o, b.io[1], p[1]);
+    ld_vec<VX>(vel, o, 1.f, p[2]);
+    ld_vec<VX>(model, o, 1.f, p[3]);
+  }
+  template <int VX>
+  __device__ void storev(const Blk2 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
+    st_vec<VX>(s0, o, out[0], b.sn[0]);
+    st_vec<VX>(r0, o, out[1], b.sn[1]);
+    if (hist) st_vec<VX>(hist, o, out[2], b.sh);
   }
   __device__ void cell(const Blk2 &b, const SrcPln &pl, const CellArgs &c,
-                       const float *lap, const float *ctr, const float (&p)[NPRE],
-                       float (&tmax)[NF]) const {
+                       const float *lap, const float *ctr, const float *p,
+                       float (&out)[NOUT], float (&tmax)[NF]) const {
     const float v = p[2], bb = c.b;
     float sn = bb * (v * lap[0] + ctr[0] + ctr[0] - bb * p[0]);
     if (pl.pm) sn = add_points(pts, pl.pm, c.x, c.y, sn, amp);
@@ -146,9 +154,9 @@ struct FwdOp {
       sig = bb * lap[0];
       rn = rn + p[3] * sig;
     }
-    st_level(s0, c.o, sn, b.sn[0]);
-    st_level(r0, c.o, rn, b.sn[1]);
-    if (hist) st_level(hist, c.o, sig, b.sh);
+    out[0] = sn;
+    out[1] = rn;
+    out[2] = sig;
     if constexpr (H) {
       tmax[0] = fmaxf(tmax[0], fabsf(sn));
       tmax[1] = fmaxf(tmax[1], fabsf(rn));
@@ -167,7 +175,7 @@ struct FwdOp {
 
 // ------------------------------------------------------------- sourceProp
 struct SrcOp {
-  static constexpr int NF = 1, NPRE = 2, NMAX = 0;
+  static constexpr int NF = 1, NPRE = 2, NOUT = 1, NMAX = 0;
   static constexpr bool DAMP = false;
   const float *__restrict__ s1;
   float *__restrict__ s0;
@@ -177,25 +185,34 @@ struct SrcOp {
   __device__ uint32_t block(int x0, int x1, int y0, int y1, int z0, int z1) const {
     return points_in_block(pts, x0, x1, y0, y1, z0, z1);
   }
-  __device__ float ld(uint32_t, int, uint32_t o) const { return at(s1, o); }
+  template <int VX>
+  __device__ void ldv(uint32_t, int, uint32_t o, float (&v)[VX]) const {
+    ld_vec<VX>(s1, o, 1.f, v);
+  }
   __device__ uint32_t plane(uint32_t blk, int z) const {
     return points_on_plane(pts, blk, z);
   }
-  __device__ void prefetch(uint32_t, float (&p)[NPRE], uint32_t o) const {
-    p[0] = at(s0, o);
-    p[1] = at(vel, o);
+  template <int VX>
+  __device__ void prefetchv(uint32_t, float (&p)[NPRE][VX], uint32_t o) const {
+    ld_vec<VX>(s0, o, 1.f, p[0]);
+    ld_vec<VX>(vel, o, 1.f, p[1]);
+  }
+  template <int VX>
+  __device__ void storev(uint32_t, uint32_t o, const float (&out)[NOUT][VX]) const {
+    st_vec<VX>(s0, o, out[0], 1.f);
   }
   __device__ void cell(uint32_t, uint32_t pm, const CellArgs &c, const float *lap,
-                       const float *ctr, const float (&p)[NPRE], float (&)[NF]) const {
+                       const float *ctr, const float *p, float (&out)[NOUT],
+                       float (&)[NF]) const {
     float sn = p[1] * lap[0] + ctr[0] + ctr[0] - p[0];
     if (pm) sn = add_points(pts, pm, c.x, c.y, sn, amp);
-    at(s0, c.o) = sn;
+    out[0] = sn;
   }
 };
 
 // ------------------------------------------------------ reference adjoint
 struct RadjOp {
-  static constexpr int NF = 2, NPRE = 5, NMAX = 0;
+  static constexpr int NF = 2, NPRE = 5, NOUT = 3, NMAX = 0;
   static constexpr bool DAMP = true;
   const float *__restrict__ s1;
   const float *__restrict__ r1;
@@ -219,25 +236,35 @@ struct RadjOp {
   __device__ uint32_t block(int x0, int x1, int y0, int y1, int z0, int z1) const {
     return pre ? points_in_block(pts, x0, x1, y0, y1, z0, z1) : 0u;
   }
-  __device__ float ld(uint32_t, int f, uint32_t o) const { return at(f == 0 ? s1 : r1, o); }
-  __device__ SrcPln plane(uint32_t blk, int z) const {
-    return {points_on_plane(pts, blk, z), rm.plane_slot[z]};
+  template <int VX>
+  __device__ void ldv(uint32_t, int f, uint32_t o, float (&v)[VX]) const {
+    ld_vec<VX>(f == 0 ? s1 : r1, o, 1.f, v);
   }
-  __device__ void prefetch(uint32_t, float (&p)[NPRE], uint32_t o) const {
-    p[0] = at(s0, o);
-    p[1] = at(r0, o);
-    p[2] = at(vel1, o);
-    p[3] = at(vel2, o);
-    p[4] = at(img, o);
+  __device__ SrcPln plane(uint32_t blk, int z) const {
+    return {points_on_plane(pts, blk, z), ldc(rm.plane_slot, z)};
+  }
+  template <int VX>
+  __device__ void prefetchv(uint32_t, float (&p)[NPRE][VX], uint32_t o) const {
+    ld_vec<VX>(s0, o, 1.f, p[0]);
+    ld_vec<VX>(r0, o, 1.f, p[1]);
+    ld_vec<VX>(vel1, o, 1.f, p[2]);
+    ld_vec<VX>(vel2, o, 1.f, p[3]);
+    ld_vec<VX>(img, o, 1.f, p[4]);
+  }
+  template <int VX>
+  __device__ void storev(uint32_t, uint32_t o, const float (&out)[NOUT][VX]) const {
+    if (s_prop) st_vec<VX>(s0, o, out[0], 1.f);
+    st_vec<VX>(r0, o, out[1], 1.f);
+    st_vec<VX>(img, o, out[2], 1.f);
   }
   __device__ void cell(uint32_t, const SrcPln &pl, const CellArgs &c, const float *lap,
-                       const float *ctr, const float (&p)[NPRE], float (&)[NF]) const {
+                       const float *ctr, const float *p, float (&out)[NOUT],
+                       float (&)[NF]) const {
     float sn;
     if (s_prop) {
       float so = p[0];
       if (pl.pm) so = add_points(pts, pl.pm, c.x, c.y, so, amp_pre);
       sn = p[2] * lap[0] + ctr[0] + ctr[0] - so;
-      at(s0, c.o) = sn;
     } else {
       sn = p[0];
     }
@@ -248,8 +275,9 @@ struct RadjOp {
         rn = __fadd_rn(rn, __fmul_rn(sc, sinc8(tab, tr + (long long)id * rm.nrec + i,
                                                rm.nrec)));
     }
-    at(r0, c.o) = rn;
-    at(img, c.o) = p[4] + sn * rn;
+    out[0] = sn;
+    out[1] = rn;
+    out[2] = p[4] + sn * rn;
   }
 };
 
@@ -257,7 +285,7 @@ struct RadjOp {
 template <int SCATTER, class T>
 struct SrecOp {
   static constexpr bool H = kHalf<T>;
-  static constexpr int NF = 1, NPRE = 2, NMAX = H ? 1 : 0;
+  static constexpr int NF = 1, NPRE = 2, NOUT = 2, NMAX = H ? 1 : 0;
   static constexpr bool DAMP = true;
   const T *__restrict__ s1;
   T *__restrict__ s0;
@@ -292,22 +320,32 @@ struct SrecOp {
   __device__ void publish_max(int, float m) const {
     atomicMax(&ring.maxb[sl.nxt], __float_as_uint(m));
   }
-  __device__ float ld(const Blk1 &b, int, uint32_t o) const { return ld_level(s1, o, b.ic); }
+  template <int VX>
+  __device__ void ldv(const Blk1 &b, int, uint32_t o, float (&v)[VX]) const {
+    ld_vec<VX>(s1, o, b.ic, v);
+  }
   __device__ uint32_t plane(const Blk1 &b, int z) const {
     return points_on_plane(pts, b.pm, z);
   }
-  __device__ void prefetch(const Blk1 &b, float (&p)[NPRE], uint32_t o) const {
-    p[0] = ld_level(s0, o, b.io);
-    p[1] = at(vel, o);
+  template <int VX>
+  __device__ void prefetchv(const Blk1 &b, float (&p)[NPRE][VX], uint32_t o) const {
+    ld_vec<VX>(s0, o, b.io, p[0]);
+    ld_vec<VX>(vel, o, 1.f, p[1]);
   }
-  __device__ void cell(const Blk1 &b, uint32_t pm, const CellArgs &c, const float *lap,
-                       const float *ctr, const float (&p)[NPRE], float (&tmax)[NF]) const {
+  template <int VX>
+  __device__ void storev(const Blk1 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
+    st_vec<VX>(s0, o, out[0], b.sn);
+    if (sig) st_vec<VX>(sig, o, out[1], b.sh);
+  }
+  __device__ void cell(const Blk1 &, uint32_t pm, const CellArgs &c, const float *lap,
+                       const float *ctr, const float *p, float (&out)[NOUT],
+                       float (&tmax)[NF]) const {
     const float bb = c.b;
     float sn = bb * (p[1] * lap[0] + ctr[0] + ctr[0] - bb * p[0]);
     if (pm) sn = add_points(pts, pm, c.x, c.y, sn, amp);
-    st_level(s0, c.o, sn, b.sn);
     if constexpr (H) tmax[0] = fmaxf(tmax[0], fabsf(sn));
-    if (sig) st_level(sig, c.o, SCATTER == 0 ? sn : bb * lap[0], b.sh);
+    out[0] = sn;
+    out[1] = SCATTER == 0 ? sn : bb * lap[0];
   }
 };
 
@@ -317,7 +355,7 @@ struct SrecOp {
 template <class T>
 struct LamOp {
   static constexpr bool H = kHalf<T>;
-  static constexpr int NF = 1, NPRE = 4, NMAX = H ? 1 : 0;
+  static constexpr int NF = 1, NPRE = 4, NOUT = 2, NMAX = H ? 1 : 0;
   static constexpr bool DAMP = true;
   const T *__restrict__ u1;
   T *__restrict__ u0;
@@ -356,16 +394,26 @@ struct LamOp {
   __device__ void publish_max(int, float m) const {
     atomicMax(&ring.maxb[sl.nxt], __float_as_uint(m));
   }
-  __device__ float ld(const Blk1 &b, int, uint32_t o) const { return ld_level(u1, o, b.ic); }
-  __device__ int plane(const Blk1 &, int z) const { return rm.plane_slot[z]; }
-  __device__ void prefetch(const Blk1 &b, float (&p)[NPRE], uint32_t o) const {
-    p[0] = ld_level(u0, o, b.io);
-    p[1] = at(vel, o);
-    p[2] = ld_level(sig, o, b.sh);
-    p[3] = at(acc, o);
+  template <int VX>
+  __device__ void ldv(const Blk1 &b, int, uint32_t o, float (&v)[VX]) const {
+    ld_vec<VX>(u1, o, b.ic, v);
   }
-  __device__ void cell(const Blk1 &b, int slot, const CellArgs &c, const float *lap,
-                       const float *ctr, const float (&p)[NPRE], float (&tmax)[NF]) const {
+  __device__ int plane(const Blk1 &, int z) const { return ldc(rm.plane_slot, z); }
+  template <int VX>
+  __device__ void prefetchv(const Blk1 &b, float (&p)[NPRE][VX], uint32_t o) const {
+    ld_vec<VX>(u0, o, b.io, p[0]);
+    ld_vec<VX>(vel, o, 1.f, p[1]);
+    ld_vec<VX>(sig, o, b.sh, p[2]);
+    ld_vec<VX>(acc, o, 1.f, p[3]);
+  }
+  template <int VX>
+  __device__ void storev(const Blk1 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
+    st_vec<VX>(u0, o, out[0], b.sn);
+    st_vec<VX>(acc, o, out[1], 1.f);
+  }
+  __device__ void cell(const Blk1 &, int slot, const CellArgs &c, const float *lap,
+                       const float *ctr, const float *p, float (&out)[NOUT],
+                       float (&tmax)[NF]) const {
     const float bb = c.b, v = p[1];
     float un = bb * (v * lap[0] + ctr[0] + ctr[0] - bb * p[0]);
     if (slot >= 0 && inject) {
@@ -374,9 +422,9 @@ struct LamOp {
         g += sinc8(tab, tr + (long long)id * rm.nrec + i, rm.nrec);
       un += v * bb * g;
     }
-    st_level(u0, c.o, un, b.sn);
     if constexpr (H) tmax[0] = fmaxf(tmax[0], fabsf(un));
-    at(acc, c.o) = p[3] + p[2] * un;
+    out[0] = un;
+    out[1] = p[3] + p[2] * un;
   }
 };
 
@@ -390,16 +438,16 @@ struct BatchFwd {
   const float *sponge;
   float *rings;
   const ShotDesc *__restrict__ desc;
-  int step;
+  int step, npts_total;
   struct Shot {
     Grid g;
     Sponge sp;
     Op op;
   };
-  __device__ Shot shot(int s, int ty, int zc) const {
+  __device__ Shot shot(int s, int tw, int ty, int zc) const {
     const ShotDesc d = desc[s];
     Shot r;
-    r.g = tile_grid(d.nx, d.ny, d.nz, ty, zc);
+    r.g = tile_grid(d.nx, d.ny, d.nz, tw, ty, zc);
     r.sp.bx = sponge + d.sp0;
     r.sp.by = sponge + d.sp0 + d.nx;
     r.sp.bz = sponge + d.sp0 + d.nx + d.ny;
@@ -414,7 +462,7 @@ struct BatchFwd {
     o.hist = nullptr;
     o.pts.p = base.pts.p + d.pts0;
     o.pts.n = d.npts;
-    o.amp = base.amp + d.amp0 + (long long)step * d.npts;
+    o.amp = base.amp + (long long)step * npts_total + d.pts0;
     o.rm.plane_slot = base.rm.plane_slot + d.slot0;
     o.rm.rmap = base.rm.rmap + d.rmap0;
     o.rm.rnext = base.rm.rnext + d.rnext0;
@@ -472,12 +520,12 @@ __global__ void fill_kernel(float *p, long long n, float v) {
     p[i] = v;
 }
 
-// amp[s*npts + i] = scale * sinc8(tab[ii], src[i][id..id+7]) with
+// amp[s*stride + i] = scale * sinc8(tab[ii], src[i][id..id+7]) with
 // tt = s + t_off, id = tt / jt, ii = tt - id*jt; zero when id + 7 >= ntsrc
 // (the guard of cpu_prop.cc:167).
 __global__ void src_amp_kernel(float *amp, int nsteps, int npts, const float *src,
                                int ntsrc, const float *tab, int nsinc, int jt, int ntab,
-                               int t_off, float scale) {
+                               int t_off, float scale, int stride) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nsteps * npts) return;
   const int s = t / npts, i = t % npts;
@@ -487,7 +535,7 @@ __global__ void src_amp_kernel(float *amp, int nsteps, int npts, const float *sr
   float a = 0.f;
   if (id + 7 < ntsrc && ii < ntab)
     a = __fmul_rn(scale, sinc8(tab + (long long)ii * nsinc, src + (long long)i * ntsrc + id, 1));
-  amp[t] = a;
+  amp[(long long)s * stride + i] = a;
 }
 
 // out[i] += scale * acc[i] / (vel[i] * b[i]) on the interior.
@@ -522,17 +570,20 @@ __global__ void transpose_kernel(float *dst, const float *src, int n0, int n1) {
 
 // ------------------------------------------------------------- launchers
 
-const TileShape kShapes[kNumShapes] = {{2, 4}, {4, 4}, {2, 8}, {2, 16}, {4, 8}, {3, 8}};
+const TileShape kShapes[kNumShapes] = {{2, 4, 1}, {4, 4, 1}, {2, 8, 1}, {2, 16, 1},
+                                       {4, 8, 1}, {3, 8, 1}, {1, 4, 2}, {1, 4, 1},
+                                       {1, 8, 2}, {1, 8, 1}, {1, 12, 2}, {1, 16, 1}};
 
 static Grid make_grid(const Geometry &geo, const TileShape &ts, int zc) {
-  return tile_grid(geo.nx, geo.ny, geo.nz, ts.ry * ts.wy, zc > 0 ? zc : 32);
+  return tile_grid(geo.nx, geo.ny, geo.nz, kTX * ts.vx, ts.ry * ts.wy, zc > 0 ? zc : 32);
 }
 
-template <class Op, int RY, int WY>
+template <class Op, int RY, int WY, int VX>
 static hipError_t launch_shape(const Geometry &geo, int zc, const Op &op, hipStream_t st) {
-  const Grid g = make_grid(geo, {RY, WY}, zc);
+  if (VX == 2 && geo.nx % 2) return hipErrorInvalidValue;
+  const Grid g = make_grid(geo, {RY, WY, VX}, zc);
   const int nb = g.ntx * g.nty * g.ntz;
-  hipLaunchKernelGGL((march_kernel<Op, RY, WY>), dim3(nb), dim3(kTX * WY), 0, st, g,
+  hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX>), dim3(nb), dim3(kTX * WY), 0, st, g,
                      geo.taps, geo.sponge, op);
   return hipGetLastError();
 }
@@ -540,12 +591,18 @@ static hipError_t launch_shape(const Geometry &geo, int zc, const Op &op, hipStr
 template <class Op>
 static hipError_t launch(const Geometry &geo, const Tune &tu, const Op &op, hipStream_t st) {
   switch (tu.shape) {
-    case 0: return launch_shape<Op, 2, 4>(geo, tu.zchunk, op, st);
-    case 1: return launch_shape<Op, 4, 4>(geo, tu.zchunk, op, st);
-    case 2: return launch_shape<Op, 2, 8>(geo, tu.zchunk, op, st);
-    case 3: return launch_shape<Op, 2, 16>(geo, tu.zchunk, op, st);
-    case 4: return launch_shape<Op, 4, 8>(geo, tu.zchunk, op, st);
-    case 5: return launch_shape<Op, 3, 8>(geo, tu.zchunk, op, st);
+    case 0: return launch_shape<Op, 2, 4, 1>(geo, tu.zchunk, op, st);
+    case 1: return launch_shape<Op, 4, 4, 1>(geo, tu.zchunk, op, st);
+    case 2: return launch_shape<Op, 2, 8, 1>(geo, tu.zchunk, op, st);
+    case 3: return launch_shape<Op, 2, 16, 1>(geo, tu.zchunk, op, st);
+    case 4: return launch_shape<Op, 4, 8, 1>(geo, tu.zchunk, op, st);
+    case 5: return launch_shape<Op, 3, 8, 1>(geo, tu.zchunk, op, st);
+    case 6: return launch_shape<Op, 1, 4, 2>(geo, tu.zchunk, op, st);
+    case 7: return launch_shape<Op, 1, 4, 1>(geo, tu.zchunk, op, st);
+    case 8: return launch_shape<Op, 1, 8, 2>(geo, tu.zchunk, op, st);
+    case 9: return launch_shape<Op, 1, 8, 1>(geo, tu.zchunk, op, st);
+    case 10: return launch_shape<Op, 1, 12, 2>(geo, tu.zchunk, op, st);
+    case 11: return launch_shape<Op, 1, 16, 1>(geo, tu.zchunk, op, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -661,11 +718,12 @@ hipError_t launch_fill(float *p, long long n, float v, hipStream_t st) {
 
 hipError_t launch_src_amp(float *amp, int nsteps, int npts, const float *src, int ntsrc,
                           const float *tab, int nsinc, int jt, int ntab, int t_off,
-                          float scale, hipStream_t st) {
+                          float scale, hipStream_t st, int stride) {
   const int n = nsteps * npts;
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(src_amp_kernel, dim3((n + 255) / 256), dim3(256), 0, st, amp, nsteps,
-                     npts, src, ntsrc, tab, nsinc, jt, ntab, t_off, scale);
+                     npts, src, ntsrc, tab, nsinc, jt, ntab, t_off, scale,
+                     stride < 0 ? npts : stride);
   return hipGetLastError();
 }
 
@@ -679,7 +737,7 @@ hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
   return hipGetLastError();
 }
 
-template <class BOp, int RY, int WY>
+template <class BOp, int RY, int WY, int VX>
 static hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdStep &b,
                                      const BOp &op, hipStream_t st) {
   if (b.nshots < 1 || b.nshots > kMaxBatch) return hipErrorInvalidValue;
@@ -689,13 +747,14 @@ static hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdSte
   int tiles = 0;
   for (int s = 0; s < b.nshots; s++) {
     const ShotDesc &d = b.host_desc[s];
-    const Grid g = tile_grid(d.nx, d.ny, d.nz, RY * WY, bt.zc);
+    if (VX == 2 && d.nx % 2) return hipErrorInvalidValue;
+    const Grid g = tile_grid(d.nx, d.ny, d.nz, kTX * VX, RY * WY, bt.zc);
     bt.tile0[s] = tiles;
     tiles += g.ntx * g.nty * g.ntz;
   }
   bt.tile0[b.nshots] = tiles;
-  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY>), dim3(tiles), dim3(kTX * WY), 0, st,
-                     bt, taps, op);
+  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX>), dim3(tiles), dim3(kTX * WY), 0,
+                     st, bt, taps, op);
   return hipGetLastError();
 }
 
@@ -703,12 +762,18 @@ template <class BOp>
 static hipError_t launch_batch(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
                                const BOp &op, hipStream_t st) {
   switch (tu.shape) {
-    case 0: return launch_batch_shape<BOp, 2, 4>(taps, tu.zchunk, b, op, st);
-    case 1: return launch_batch_shape<BOp, 4, 4>(taps, tu.zchunk, b, op, st);
-    case 2: return launch_batch_shape<BOp, 2, 8>(taps, tu.zchunk, b, op, st);
-    case 3: return launch_batch_shape<BOp, 2, 16>(taps, tu.zchunk, b, op, st);
-    case 4: return launch_batch_shape<BOp, 4, 8>(taps, tu.zchunk, b, op, st);
-    case 5: return launch_batch_shape<BOp, 3, 8>(taps, tu.zchunk, b, op, st);
+    case 0: return launch_batch_shape<BOp, 2, 4, 1>(taps, tu.zchunk, b, op, st);
+    case 1: return launch_batch_shape<BOp, 4, 4, 1>(taps, tu.zchunk, b, op, st);
+    case 2: return launch_batch_shape<BOp, 2, 8, 1>(taps, tu.zchunk, b, op, st);
+    case 3: return launch_batch_shape<BOp, 2, 16, 1>(taps, tu.zchunk, b, op, st);
+    case 4: return launch_batch_shape<BOp, 4, 8, 1>(taps, tu.zchunk, b, op, st);
+    case 5: return launch_batch_shape<BOp, 3, 8, 1>(taps, tu.zchunk, b, op, st);
+    case 6: return launch_batch_shape<BOp, 1, 4, 2>(taps, tu.zchunk, b, op, st);
+    case 7: return launch_batch_shape<BOp, 1, 4, 1>(taps, tu.zchunk, b, op, st);
+    case 8: return launch_batch_shape<BOp, 1, 8, 2>(taps, tu.zchunk, b, op, st);
+    case 9: return launch_batch_shape<BOp, 1, 8, 1>(taps, tu.zchunk, b, op, st);
+    case 10: return launch_batch_shape<BOp, 1, 12, 2>(taps, tu.zchunk, b, op, st);
+    case 11: return launch_batch_shape<BOp, 1, 16, 1>(taps, tu.zchunk, b, op, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -726,6 +791,7 @@ hipError_t launch_batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFw
     op.extract = a.extract;
     op.sl = a.h.sl; op.tapsum = a.h.tapsum; op.hscale = nullptr;
     bop.sponge = b.sponge; bop.rings = b.rings; bop.desc = b.desc; bop.step = b.step;
+    bop.npts_total = b.npts_total;
     return launch_batch(taps, tu, b, bop, st);
   };
   if (a.h.on) {

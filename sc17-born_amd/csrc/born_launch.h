@@ -14,11 +14,12 @@ struct Geometry {
   Sponge sponge;
 };
 
-// Tile shape: RY rows per thread, WY wavefronts per block (tile 64 x RY*WY).
+// Tile shape: RY rows per thread, WY wavefronts per block, VX cells per lane
+// along x (tile 64*VX x RY*WY).  VX = 2 needs an even nx.
 struct TileShape {
-  int ry, wy;
+  int ry, wy, vx;
 };
-constexpr int kNumShapes = 6;
+constexpr int kNumShapes = 12;
 extern const TileShape kShapes[kNumShapes];
 
 // Launch configuration of one step kernel: tile shape and z planes per block.
@@ -118,22 +119,21 @@ struct ShotDesc {
   long long cell0;  // fields, velocity and model
   long long rmap0;  // receiver map
   long long tr0;    // traces, time-major [ntb][nrec]
-  long long amp0;   // source amplitudes [nt][npts]
   int sp0;          // sponge tables: x at sp0, y at sp0 + nx, z at sp0 + nx + ny
-  int pts0;         // source points
+  int pts0;         // source points (and amplitude columns)
   int slot0;        // receiver plane slots (nz entries)
   int rnext0;       // receiver chains
   int nrec;
   int ring0;        // FP16 scales: ring s (8 words), ring r (8), max |model| (1)
   float gain;       // FP16 bound gain: 2 + max(V) * tapsum
-  int pad;
 };
 constexpr int kRingWords = 32;  // ring words per shot
 
 // One step of a batch: a.* hold the arrays of the whole batch (shot 0 at
 // offset 0), moved per shot by the descriptors.
 struct BatchFwdStep {
-  FwdStep a;
+  FwdStep a;  // a.amp: amplitudes [step][npts_total], shot s at column pts0
+  int npts_total;
   const float *sponge;
   float *rings;
   const ShotDesc *desc;      // device copy
@@ -158,7 +158,7 @@ hipError_t launch_lam_step(const Geometry &geo, const Tune &tu, const LamStep &a
 hipError_t launch_fill(float *p, long long n, float v, hipStream_t st);
 hipError_t launch_src_amp(float *amp, int nsteps, int npts, const float *src, int ntsrc,
                           const float *tab, int nsinc, int jt, int ntab, int t_off,
-                          float scale, hipStream_t st);
+                          float scale, hipStream_t st, int stride = -1);
 hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
                              const float *vel, float scale, hipStream_t st);
 hipError_t launch_transpose(float *dst, const float *src, int n0, int n1, hipStream_t st);

@@ -59,7 +59,15 @@ SIGNATURES = {
     "born_set_autotune": (_i, [_vp, _i]),
     "born_get_tuning": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
                              C.POINTER(C.c_float)]),
+    "born_get_tile": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i)]),
+    "born_set_tuning": (_i, [_vp, _i, _i, _i, _i, _i]),
     "born_get_stream": (_vp, [_vp]),
+    "born_batch_create": (_i, [_vp, _i, _vp, _f, _f, _f, C.POINTER(_vp)]),
+    "born_batch_destroy": (_i, [_vp]),
+    "born_batch_forward": (_i, [_vp, _i, _i, _i]),
+    "born_batch_traces": (_i, [_vp, _i, _vp]),
+    "born_batch_get_tile": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
+                                 C.POINTER(C.c_float)]),
     "born_synchronize": (_i, [_vp]),
 }
 
@@ -236,9 +244,17 @@ class Propagator:
             ry, wy, zc, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
             self._chk(self.lib.born_get_tuning(self.h, k, C.byref(ry), C.byref(wy), C.byref(zc),
                                                C.byref(ms)), "tuning")
-            out[name] = {"tile": [64, ry.value * wy.value], "rows_per_thread": ry.value,
-                         "waves": wy.value, "zchunk": zc.value, "ms": round(ms.value, 4)}
+            tx, tyy = C.c_int(), C.c_int()
+            self._chk(self.lib.born_get_tile(self.h, k, C.byref(tx), C.byref(tyy)), "tile")
+            out[name] = {"tile": [tx.value, tyy.value], "rows_per_thread": ry.value,
+                         "waves": wy.value, "cells_per_lane": tx.value // 64,
+                         "zchunk": zc.value, "ms": round(ms.value, 4)}
         return out
+
+    def set_tuning(self, kernel, rows_per_thread, waves, cells_per_lane, zchunk):
+        k = self.KERNELS.index(kernel)
+        self._chk(self.lib.born_set_tuning(self.h, k, rows_per_thread, waves, cells_per_lane,
+                                           zchunk), "set_tuning")
 
     @property
     def stream(self):
@@ -389,3 +405,68 @@ class BornOp:
         res = (C.c_double * max(niter, 1))()
         self._chk(self.lib.born_op_lsrtm(self.h, _ptr(d), _ptr(m), niter, res), "lsrtm")
         return m, list(res)[:niter]
+
+
+class _Shot(C.Structure):
+    """born_shot (include/born_hip.h)"""
+    _fields_ = [("nx", _i), ("ny", _i), ("nz", _i), ("vel", _vp), ("model", _vp),
+                ("npts", _i), ("nt_big", _i), ("src_locs", _vp), ("src_vals", _vp),
+                ("rec_nx", _i), ("rec_ny", _i), ("rec_locs", _vp)]
+
+
+class Batch:
+    """Shots with their own windowed domains run together (born_batch_*).
+
+    shots: dicts with nx, ny, nz, vel ([nz][ny][nx] V = v^2 dt^2), model,
+    src_locs, src_vals ([npts][nt_big]), rec_nx, rec_ny, rec_locs.  The
+    propagator supplies the sinc tables, sponge, storage and stream."""
+
+    def __init__(self, prop, shots, d):
+        self.prop = prop
+        self.lib = prop.lib
+        self._keep = []
+        arr = (_Shot * len(shots))()
+        self.nrec, self.ntb = [], []
+        for k, sh in enumerate(shots):
+            vel = np.ascontiguousarray(sh["vel"], np.float32)
+            model = np.ascontiguousarray(sh["model"], np.float32)
+            sl = np.ascontiguousarray(sh["src_locs"], np.int32)
+            sv = np.ascontiguousarray(sh["src_vals"], np.float32)
+            rl = np.ascontiguousarray(sh["rec_locs"], np.int32)
+            self._keep += [vel, model, sl, sv, rl]
+            arr[k] = _Shot(sh["nx"], sh["ny"], sh["nz"], _ptr(vel), _ptr(model), len(sl),
+                           sv.shape[-1], _ptr(sl), _ptr(sv), sh["rec_nx"], sh["rec_ny"],
+                           _ptr(rl))
+            self.nrec.append(sh["rec_nx"] * sh["rec_ny"])
+            self.ntb.append(sv.shape[-1])
+        h = _vp()
+        d1, d2, d3 = d
+        prop._chk(self.lib.born_batch_create(prop.h, len(shots), arr, d1, d2, d3, C.byref(h)),
+                  "batch_create")
+        self.h = h
+        self._keep = []  # copied to the device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.born_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, jt, nt, scatter=0):
+        self.prop._chk(self.lib.born_batch_forward(self.h, jt, nt, scatter), "batch_forward")
+
+    def traces(self, shot):
+        out = np.zeros((self.nrec[shot], self.ntb[shot]), np.float32)
+        self.prop._chk(self.lib.born_batch_traces(self.h, shot, _ptr(out)), "batch_traces")
+        return out
+
+    def tile(self, scatter=0):
+        tx, ty, zc, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        self.prop._chk(self.lib.born_batch_get_tile(self.h, scatter, C.byref(tx), C.byref(ty),
+                                                    C.byref(zc), C.byref(ms)), "batch_tile")
+        return {"tile": [tx.value, ty.value], "zchunk": zc.value, "ms": round(ms.value, 4)}
