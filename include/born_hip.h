@@ -173,14 +173,15 @@ int born_set_autotune(born_ctx *ctx, int on);
 int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
                     int *waves_per_block, int *zchunk, float *ms);
 
-/* The chosen tile of a step kernel in cells: tile_x = 64 * cells per lane
- * (64 or 128), tile_y = rows per thread * waves per block. */
-int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y);
+/* The chosen tile of a step kernel: tile_x x tile_y cells of the xy plane,
+ * planes_per_step z planes advanced between two waits for memory. */
+int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y,
+                  int *planes_per_step);
 /* Fix the launch shape of a step kernel (rows per thread x waves per block
- * x cells per lane, one of the autotuner's shapes; 2 cells per lane needs
- * an even nx) and its z chunk; the autotuner then leaves that kernel. */
+ * x planes per step, one of the autotuner's shapes) and its z chunk; the
+ * autotuner then leaves that kernel. */
 int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_per_block,
-                    int cells_per_lane, int zchunk);
+                    int planes_per_step, int zchunk);
 
 void *born_get_stream(const born_ctx *ctx);
 

@@ -1252,21 +1252,23 @@ int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
   return BORN_OK;
 }
 
-int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y) {
+int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y,
+                  int *planes_per_step) {
   if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
   const born::TileShape &ts = born::kShapes[tuning(ctx, kernel).shape];
   if (tile_x) *tile_x = born::kTX * ts.vx;
   if (tile_y) *tile_y = ts.ry * ts.wy;
+  if (planes_per_step) *planes_per_step = ts.pz;
   return BORN_OK;
 }
 
 int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_per_block,
-                    int cells_per_lane, int zchunk) {
+                    int planes_per_step, int zchunk) {
   if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
   if (zchunk < 1) return fail(ctx, BORN_ERR_ARG, "zchunk must be positive");
   for (int sh = 0; sh < born::kNumShapes; sh++) {
     const born::TileShape &ts = born::kShapes[sh];
-    if (ts.ry == rows_per_thread && ts.wy == waves_per_block && ts.vx == cells_per_lane) {
+    if (ts.ry == rows_per_thread && ts.wy == waves_per_block && ts.pz == planes_per_step) {
       ctx->tune[kernel] = {sh, zchunk};
       ctx->tune_ms[kernel] = 0.f;  // fixed: the autotuner leaves it
       return BORN_OK;

@@ -249,7 +249,7 @@ __device__ __forceinline__ void st_vec(__half *p, uint32_t o, const float (&v)[V
 //             float (&tmax)[NF]);                the new values of one cell
 //   storev<VX>(const Blk&, uint32_t o, const float (&out)[NOUT][VX])
 //   void publish_max(int f, float m);           (NMAX > 0) block max of |new|
-template <class Op, int RY, int WY, int VX>
+template <class Op, int RY, int WY, int VX, int PZ>
 __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const Sponge &sp,
                                            const Op &op, const int L) {
   constexpr int NF = Op::NF;
@@ -259,7 +259,8 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   constexpr int TY = RY * WY;
   constexpr int LW = TW + 2 * kH;
   constexpr int LH = TY + 2 * kH;
-  __shared__ __align__(16) float tile[NF][LH][LW];
+  constexpr int NQ = 2 * kH + PZ;  // queue: planes z - kH .. z + kH + PZ - 1
+  __shared__ __align__(16) float tile[PZ][NF][LH][LW];
 
   const int bx = L % g.ntx;
   const int rest = L / g.ntx;
@@ -327,20 +328,22 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     hoff[k] = (uint32_t)gy * g.nx + gx;
   }
 
-  // One plane of loads ahead: the next plane's queue lead, LDS halo and
-  // pointwise inputs are issued before this plane is computed.  (Stores and
-  // loads share one counter on this architecture, so with the plane's stores
-  // pending any wait for a load drains every load issued so far: a deeper
-  // register pipeline would not keep more loads in flight.)
+  // The tile marches PZ planes per step: the loads of the next PZ planes
+  // (queue leads, LDS halos, pointwise inputs) are issued before this step's
+  // PZ planes are computed, and one wait and two barriers serve PZ planes.
+  // (Stores and loads share one counter on this architecture, so with stores
+  // pending any wait for a load drains every load issued so far; the loads
+  // in flight per wait are raised by taking more planes per step, not by a
+  // deeper register pipeline.)
+  //
   // The LDS halo of a plane is loaded together with the plane's centre rows
-  // (when the plane enters the queue as its lead), not kH planes later: the
+  // (when the plane enters the queue as a lead), not kH planes later: the
   // neighbouring tiles read those rows at about the same time, so they come
-  // from L2; kH planes later the L2 has turned over and they would be
-  // fetched from HBM again.  hq[k] holds the halo of plane z + k.
-  float q[NF][RY][VX][2 * kH + 1];
-  float lead[NF][RY][VX];
-  float hq[kH + 1][NF][HPT][VX];
-  float pre[RY][NP][VX], pre_n[RY][NP][VX];
+  // from L2.  hq[k] holds the halo of plane z + k.
+  float q[NF][RY][VX][NQ];
+  float lead[PZ][NF][RY][VX];
+  float hq[kH + PZ][NF][HPT][VX];
+  float pre[PZ][RY][NP][VX], pre_n[PZ][RY][NP][VX];
   float tmax[NF];
 #pragma unroll
   for (int f = 0; f < NF; f++) {
@@ -354,10 +357,13 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
 #pragma unroll
         for (int v = 0; v < VX; v++) q[f][j][v][k] = w[v];
       }
-      op.template ldv<VX>(blk, f, (uint32_t)(z0 + kH) * g.plane + row[j], lead[f][j]);
+#pragma unroll
+      for (int p = 0; p < PZ; p++)
+        op.template ldv<VX>(blk, f, (uint32_t)min(z0 + kH + p, g.nz - 1) * g.plane + row[j],
+                            lead[p][f][j]);
     }
 #pragma unroll
-    for (int p = 0; p <= kH; p++) {
+    for (int p = 0; p < kH + PZ; p++) {
       const int zp = min(z0 + p, g.nz - 1);
 #pragma unroll
       for (int k = 0; k < HPT; k++) {
@@ -371,47 +377,61 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     }
   }
 #pragma unroll
-  for (int j = 0; j < RY; j++)
-    op.template prefetchv<VX>(blk, pre[j], (uint32_t)z0 * g.plane + row[j]);
-
-  // The new values of a plane are stored at the start of the next plane,
-  // after its wait and before its loads: the wait for a plane's inputs then
-  // finds stores and loads that were issued a whole plane earlier.
-  float pend[RY][NO][VX];
-  uint32_t pend_o = 0;
-  bool pend_on = false;
-  auto flush = [&]() {
-    if (pend_on) {
+  for (int p = 0; p < PZ; p++)
 #pragma unroll
-      for (int j = 0; j < RY; j++)
-        if (in[j]) op.template storev<VX>(blk, pend_o + row[j], pend[j]);
+    for (int j = 0; j < RY; j++)
+      op.template prefetchv<VX>(blk, pre[p][j],
+                                (uint32_t)min(z0 + p, g.nz - 1) * g.plane + row[j]);
+
+  // The new values of a step are stored at the start of the next step,
+  // after its wait and before its loads: the wait for a step's inputs then
+  // finds stores and loads that were issued a whole step earlier.
+  float pend[PZ][RY][NO][VX];
+  int pend_z = -1;
+  auto flush = [&]() {
+    if (pend_z >= 0) {
+#pragma unroll
+      for (int p = 0; p < PZ; p++)
+        if (pend_z + p < z1) {
+          const uint32_t po = (uint32_t)(pend_z + p) * g.plane;
+#pragma unroll
+          for (int j = 0; j < RY; j++)
+            if (in[j]) op.template storev<VX>(blk, po + row[j], pend[p][j]);
+        }
     }
   };
 
-  for (int z = z0; z < z1; ++z) {
-    const uint32_t zo = (uint32_t)z * g.plane;
+  for (int z = z0; z < z1; z += PZ) {
+    // stage the step's planes: queue leads, centre and halo into LDS
 #pragma unroll
     for (int f = 0; f < NF; f++) {
 #pragma unroll
       for (int j = 0; j < RY; j++) {
 #pragma unroll
-        for (int v = 0; v < VX; v++) q[f][j][v][2 * kH] = lead[f][j][v];
-        float *d = &tile[f][kH + ty + WY * j][kH + VX * tx];
-        if constexpr (VX == 1)
-          d[0] = q[f][j][0][kH];
-        else
-          *(float2 *)d = make_float2(q[f][j][0][kH], q[f][j][1][kH]);
-      }
+        for (int p = 0; p < PZ; p++)
 #pragma unroll
-      for (int k = 0; k < HPT; k++) {
-        if (hv[k]) {
-          float *d = &tile[f][hr[k]][hc[k]];
+          for (int v = 0; v < VX; v++) q[f][j][v][2 * kH + p] = lead[p][f][j][v];
+#pragma unroll
+        for (int p = 0; p < PZ; p++) {
+          float *d = &tile[p][f][kH + ty + WY * j][kH + VX * tx];
           if constexpr (VX == 1)
-            d[0] = hq[0][f][k][0];
+            d[0] = q[f][j][0][kH + p];
           else
-            *(float2 *)d = make_float2(hq[0][f][k][0], hq[0][f][k][1]);
+            *(float2 *)d = make_float2(q[f][j][0][kH + p], q[f][j][1][kH + p]);
         }
       }
+#pragma unroll
+      for (int p = 0; p < PZ; p++)
+#pragma unroll
+        for (int k = 0; k < HPT; k++) {
+          if (hv[k]) {
+            float *d = &tile[p][f][hr[k]][hc[k]];
+            if constexpr (VX == 1)
+              d[0] = hq[p][f][k][0];
+            else
+              *(float2 *)d = make_float2(hq[p][f][k][0], hq[p][f][k][1]);
+          }
+        }
     }
 #pragma unroll
     for (int p = 0; p < kH; p++)
@@ -420,95 +440,102 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
 #pragma unroll
         for (int k = 0; k < HPT; k++)
 #pragma unroll
-          for (int v = 0; v < VX; v++) hq[p][f][k][v] = hq[p + 1][f][k][v];
+          for (int v = 0; v < VX; v++) hq[p][f][k][v] = hq[p + PZ][f][k][v];
     flush();
-    // next plane's loads, in flight during this plane's compute: the queue
-    // lead z+1+kH with its halo, and the pointwise inputs of z+1
-    if (z + 1 < z1) {
-      const uint32_t zn = zo + g.plane;
-      const uint32_t zh = (uint32_t)min(z + 1 + kH, g.nz - 1) * g.plane;
+    // the next step's loads, in flight during this step's compute: queue
+    // leads z+PZ+kH.. with their halos, and the pointwise inputs of z+PZ..
+    if (z + PZ < z1) {
 #pragma unroll
-      for (int f = 0; f < NF; f++) {
+      for (int p = 0; p < PZ; p++) {
+        const uint32_t zl = (uint32_t)min(z + PZ + kH + p, g.nz - 1) * g.plane;
+        const uint32_t zn = (uint32_t)min(z + PZ + p, g.nz - 1) * g.plane;
 #pragma unroll
-        for (int j = 0; j < RY; j++)
-          op.template ldv<VX>(blk, f, zn + kH * g.plane + row[j], lead[f][j]);
+        for (int f = 0; f < NF; f++) {
 #pragma unroll
-        for (int k = 0; k < HPT; k++)
-          if (hv[k]) op.template ldv<VX>(blk, f, zh + hoff[k], hq[kH][f][k]);
+          for (int j = 0; j < RY; j++) op.template ldv<VX>(blk, f, zl + row[j], lead[p][f][j]);
+#pragma unroll
+          for (int k = 0; k < HPT; k++)
+            if (hv[k]) op.template ldv<VX>(blk, f, zl + hoff[k], hq[kH + p][f][k]);
+        }
+#pragma unroll
+        for (int j = 0; j < RY; j++) op.template prefetchv<VX>(blk, pre_n[p][j], zn + row[j]);
       }
-#pragma unroll
-      for (int j = 0; j < RY; j++) op.template prefetchv<VX>(blk, pre_n[j], zn + row[j]);
     }
     __syncthreads();
-    const auto pl = op.plane(blk, z);
-    const float bzv = Op::DAMP ? ldc(sp.bz, z) : 1.f;
 #pragma unroll
-    for (int j = 0; j < RY; j++) {
-      if (!in[j]) continue;
-      const int lr = kH + ty + WY * j, lc = kH + VX * tx;
-      float lap[VX][NF], ctr[VX][NF];
+    for (int p = 0; p < PZ; p++) {
+      const int zz = z + p;
+      if (PZ > 1 && zz >= z1) break;
+      const uint32_t zo = (uint32_t)zz * g.plane;
+      const auto pl = op.plane(blk, zz);
+      const float bzv = Op::DAMP ? ldc(sp.bz, zz) : 1.f;
 #pragma unroll
-      for (int f = 0; f < NF; f++) {
-        // the row from lc - 4 to lc + VX + 3, and the column pairs above/below
-        float xs[2 * kH + VX];
-        float up[kH][VX], dn[kH][VX];
-        if constexpr (VX == 1) {
+      for (int j = 0; j < RY; j++) {
+        if (!in[j]) continue;
+        const int lr = kH + ty + WY * j, lc = kH + VX * tx;
+        float lap[VX][NF], ctr[VX][NF];
 #pragma unroll
-          for (int i = 0; i < 2 * kH + 1; i++) xs[i] = tile[f][lr][lc - kH + i];
+        for (int f = 0; f < NF; f++) {
+          // the row from lc - 4 to lc + VX + 3, and the columns above/below
+          float xs[2 * kH + VX];
+          float up[kH][VX], dn[kH][VX];
+          if constexpr (VX == 1) {
 #pragma unroll
-          for (int k = 1; k <= kH; k++) {
-            up[k - 1][0] = tile[f][lr - k][lc];
-            dn[k - 1][0] = tile[f][lr + k][lc];
+            for (int i = 0; i < 2 * kH + 1; i++) xs[i] = tile[p][f][lr][lc - kH + i];
+#pragma unroll
+            for (int k = 1; k <= kH; k++) {
+              up[k - 1][0] = tile[p][f][lr - k][lc];
+              dn[k - 1][0] = tile[p][f][lr + k][lc];
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < kH + 1; i++) {
+              const float2 w = *(const float2 *)&tile[p][f][lr][lc - kH + 2 * i];
+              xs[2 * i] = w.x;
+              xs[2 * i + 1] = w.y;
+            }
+#pragma unroll
+            for (int k = 1; k <= kH; k++) {
+              const float2 a = *(const float2 *)&tile[p][f][lr - k][lc];
+              const float2 b = *(const float2 *)&tile[p][f][lr + k][lc];
+              up[k - 1][0] = a.x;
+              up[k - 1][1] = a.y;
+              dn[k - 1][0] = b.x;
+              dn[k - 1][1] = b.y;
+            }
           }
-        } else {
 #pragma unroll
-          for (int i = 0; i < kH + 1; i++) {
-            const float2 w = *(const float2 *)&tile[f][lr][lc - kH + 2 * i];
-            xs[2 * i] = w.x;
-            xs[2 * i + 1] = w.y;
-          }
+          for (int v = 0; v < VX; v++) {
+            float a = t.c0 * q[f][j][v][kH + p];
 #pragma unroll
-          for (int k = 1; k <= kH; k++) {
-            const float2 a = *(const float2 *)&tile[f][lr - k][lc];
-            const float2 b = *(const float2 *)&tile[f][lr + k][lc];
-            up[k - 1][0] = a.x;
-            up[k - 1][1] = a.y;
-            dn[k - 1][0] = b.x;
-            dn[k - 1][1] = b.y;
+            for (int k = 1; k <= kH; k++) a += t.cx[k - 1] * (xs[kH + v - k] + xs[kH + v + k]);
+#pragma unroll
+            for (int k = 1; k <= kH; k++) a += t.cy[k - 1] * (up[k - 1][v] + dn[k - 1][v]);
+#pragma unroll
+            for (int k = 1; k <= kH; k++)
+              a += t.cz[k - 1] * (q[f][j][v][kH + p - k] + q[f][j][v][kH + p + k]);
+            lap[v][f] = a;
+            ctr[v][f] = q[f][j][v][kH + p];
           }
         }
 #pragma unroll
         for (int v = 0; v < VX; v++) {
-          float a = t.c0 * q[f][j][v][kH];
+          CellArgs c;
+          c.x = x + v;
+          c.y = yy[j];
+          c.z = zz;
+          c.o = zo + row[j] + v;
+          c.b = fminf(bxy[j][v], bzv);
+          float pv[NP], o[NO];
 #pragma unroll
-          for (int k = 1; k <= kH; k++) a += t.cx[k - 1] * (xs[kH + v - k] + xs[kH + v + k]);
+          for (int k = 0; k < NP; k++) pv[k] = pre[p][j][k][v];
+          op.cell(blk, pl, c, lap[v], ctr[v], pv, o, tmax);
 #pragma unroll
-          for (int k = 1; k <= kH; k++) a += t.cy[k - 1] * (up[k - 1][v] + dn[k - 1][v]);
-#pragma unroll
-          for (int k = 1; k <= kH; k++)
-            a += t.cz[k - 1] * (q[f][j][v][kH - k] + q[f][j][v][kH + k]);
-          lap[v][f] = a;
-          ctr[v][f] = q[f][j][v][kH];
+          for (int k = 0; k < NO; k++) pend[p][j][k][v] = o[k];
         }
       }
-#pragma unroll
-      for (int v = 0; v < VX; v++) {
-        CellArgs c;
-        c.x = x + v;
-        c.y = yy[j];
-        c.z = z;
-        c.o = zo + row[j] + v;
-        c.b = fminf(bxy[j][v], bzv);
-        float p[NP], o[NO];
-#pragma unroll
-        for (int k = 0; k < NP; k++) p[k] = pre[j][k][v];
-        op.cell(blk, pl, c, lap[v], ctr[v], p, o, tmax);
-#pragma unroll
-        for (int k = 0; k < NO; k++) pend[j][k][v] = o[k];
-      }
     }
-    pend_o = zo;
-    pend_on = true;
+    pend_z = z;
     __syncthreads();
 #pragma unroll
     for (int f = 0; f < NF; f++)
@@ -517,13 +544,15 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
 #pragma unroll
         for (int v = 0; v < VX; v++)
 #pragma unroll
-          for (int k = 0; k < 2 * kH; k++) q[f][j][v][k] = q[f][j][v][k + 1];
+          for (int k = 0; k < 2 * kH; k++) q[f][j][v][k] = q[f][j][v][k + PZ];
 #pragma unroll
-    for (int j = 0; j < RY; j++)
+    for (int p = 0; p < PZ; p++)
 #pragma unroll
-      for (int k = 0; k < NP; k++)
+      for (int j = 0; j < RY; j++)
 #pragma unroll
-        for (int v = 0; v < VX; v++) pre[j][k][v] = pre_n[j][k][v];
+        for (int k = 0; k < NP; k++)
+#pragma unroll
+          for (int v = 0; v < VX; v++) pre[p][j][k][v] = pre_n[p][j][k][v];
   }
   flush();
   // block maxima of |new level| for the next step's FP16 scales
@@ -547,10 +576,10 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   }
 }
 
-template <class Op, int RY, int WY, int VX>
+template <class Op, int RY, int WY, int VX, int PZ>
 __global__ void __launch_bounds__(kTX *WY)
     march_kernel(const Grid g, const Taps t, const Sponge sp, const Op op) {
-  march_tile<Op, RY, WY, VX>(g, t, sp, op, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+  march_tile<Op, RY, WY, VX, PZ>(g, t, sp, op, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // Tiles of a batch of shots with their own grids: shot s owns the logical
@@ -579,14 +608,14 @@ __host__ __device__ inline Grid tile_grid(int nx, int ny, int nz, int tw, int ty
 // One launch advances every shot of a batch by one step.  BOp::shot(s, tw,
 // ty, zc) returns {Grid, Sponge, Op} of shot s with the op's pointers moved to
 // the shot's part of the batch arrays.
-template <class BOp, int RY, int WY, int VX>
+template <class BOp, int RY, int WY, int VX, int PZ>
 __global__ void __launch_bounds__(kTX *WY)
     march_batch_kernel(const BatchTiles bt, const Taps t, const BOp bop) {
   const int Lg = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   int s = 0;
   while (s + 1 < bt.n && Lg >= bt.tile0[s + 1]) s++;
   const auto sh = bop.shot(s, kTX * VX, RY * WY, bt.zc);
-  march_tile<typename BOp::Op, RY, WY, VX>(sh.g, t, sh.sp, sh.op, Lg - bt.tile0[s]);
+  march_tile<typename BOp::Op, RY, WY, VX, PZ>(sh.g, t, sh.sp, sh.op, Lg - bt.tile0[s]);
 }
 
 }  // namespace born

@@ -570,20 +570,20 @@ __global__ void transpose_kernel(float *dst, const float *src, int n0, int n1) {
 
 // ------------------------------------------------------------- launchers
 
-const TileShape kShapes[kNumShapes] = {{2, 4, 1}, {4, 4, 1}, {2, 8, 1}, {2, 16, 1},
-                                       {4, 8, 1}, {3, 8, 1}, {1, 4, 2}, {1, 4, 1},
-                                       {1, 8, 2}, {1, 8, 1}, {1, 12, 2}, {1, 16, 1}};
+const TileShape kShapes[kNumShapes] = {
+    {2, 4, 1, 1}, {4, 4, 1, 1}, {2, 8, 1, 1}, {2, 16, 1, 1}, {1, 8, 1, 1}, {1, 16, 1, 1},
+    {2, 4, 1, 2}, {1, 4, 1, 2}, {2, 8, 1, 2}, {1, 8, 1, 2}, {1, 12, 1, 2}, {1, 16, 1, 2}};
 
 static Grid make_grid(const Geometry &geo, const TileShape &ts, int zc) {
   return tile_grid(geo.nx, geo.ny, geo.nz, kTX * ts.vx, ts.ry * ts.wy, zc > 0 ? zc : 32);
 }
 
-template <class Op, int RY, int WY, int VX>
+template <class Op, int RY, int WY, int VX, int PZ>
 static hipError_t launch_shape(const Geometry &geo, int zc, const Op &op, hipStream_t st) {
   if (VX == 2 && geo.nx % 2) return hipErrorInvalidValue;
-  const Grid g = make_grid(geo, {RY, WY, VX}, zc);
+  const Grid g = make_grid(geo, {RY, WY, VX, PZ}, zc);
   const int nb = g.ntx * g.nty * g.ntz;
-  hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX>), dim3(nb), dim3(kTX * WY), 0, st, g,
+  hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX, PZ>), dim3(nb), dim3(kTX * WY), 0, st, g,
                      geo.taps, geo.sponge, op);
   return hipGetLastError();
 }
@@ -591,18 +591,18 @@ static hipError_t launch_shape(const Geometry &geo, int zc, const Op &op, hipStr
 template <class Op>
 static hipError_t launch(const Geometry &geo, const Tune &tu, const Op &op, hipStream_t st) {
   switch (tu.shape) {
-    case 0: return launch_shape<Op, 2, 4, 1>(geo, tu.zchunk, op, st);
-    case 1: return launch_shape<Op, 4, 4, 1>(geo, tu.zchunk, op, st);
-    case 2: return launch_shape<Op, 2, 8, 1>(geo, tu.zchunk, op, st);
-    case 3: return launch_shape<Op, 2, 16, 1>(geo, tu.zchunk, op, st);
-    case 4: return launch_shape<Op, 4, 8, 1>(geo, tu.zchunk, op, st);
-    case 5: return launch_shape<Op, 3, 8, 1>(geo, tu.zchunk, op, st);
-    case 6: return launch_shape<Op, 1, 4, 2>(geo, tu.zchunk, op, st);
-    case 7: return launch_shape<Op, 1, 4, 1>(geo, tu.zchunk, op, st);
-    case 8: return launch_shape<Op, 1, 8, 2>(geo, tu.zchunk, op, st);
-    case 9: return launch_shape<Op, 1, 8, 1>(geo, tu.zchunk, op, st);
-    case 10: return launch_shape<Op, 1, 12, 2>(geo, tu.zchunk, op, st);
-    case 11: return launch_shape<Op, 1, 16, 1>(geo, tu.zchunk, op, st);
+    case 0: return launch_shape<Op, 2, 4, 1, 1>(geo, tu.zchunk, op, st);
+    case 1: return launch_shape<Op, 4, 4, 1, 1>(geo, tu.zchunk, op, st);
+    case 2: return launch_shape<Op, 2, 8, 1, 1>(geo, tu.zchunk, op, st);
+    case 3: return launch_shape<Op, 2, 16, 1, 1>(geo, tu.zchunk, op, st);
+    case 4: return launch_shape<Op, 1, 8, 1, 1>(geo, tu.zchunk, op, st);
+    case 5: return launch_shape<Op, 1, 16, 1, 1>(geo, tu.zchunk, op, st);
+    case 6: return launch_shape<Op, 2, 4, 1, 2>(geo, tu.zchunk, op, st);
+    case 7: return launch_shape<Op, 1, 4, 1, 2>(geo, tu.zchunk, op, st);
+    case 8: return launch_shape<Op, 2, 8, 1, 2>(geo, tu.zchunk, op, st);
+    case 9: return launch_shape<Op, 1, 8, 1, 2>(geo, tu.zchunk, op, st);
+    case 10: return launch_shape<Op, 1, 12, 1, 2>(geo, tu.zchunk, op, st);
+    case 11: return launch_shape<Op, 1, 16, 1, 2>(geo, tu.zchunk, op, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -737,7 +737,7 @@ hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
   return hipGetLastError();
 }
 
-template <class BOp, int RY, int WY, int VX>
+template <class BOp, int RY, int WY, int VX, int PZ>
 static hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdStep &b,
                                      const BOp &op, hipStream_t st) {
   if (b.nshots < 1 || b.nshots > kMaxBatch) return hipErrorInvalidValue;
@@ -753,8 +753,8 @@ static hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdSte
     tiles += g.ntx * g.nty * g.ntz;
   }
   bt.tile0[b.nshots] = tiles;
-  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX>), dim3(tiles), dim3(kTX * WY), 0,
-                     st, bt, taps, op);
+  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX, PZ>), dim3(tiles), dim3(kTX * WY),
+                     0, st, bt, taps, op);
   return hipGetLastError();
 }
 
@@ -762,18 +762,18 @@ template <class BOp>
 static hipError_t launch_batch(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
                                const BOp &op, hipStream_t st) {
   switch (tu.shape) {
-    case 0: return launch_batch_shape<BOp, 2, 4, 1>(taps, tu.zchunk, b, op, st);
-    case 1: return launch_batch_shape<BOp, 4, 4, 1>(taps, tu.zchunk, b, op, st);
-    case 2: return launch_batch_shape<BOp, 2, 8, 1>(taps, tu.zchunk, b, op, st);
-    case 3: return launch_batch_shape<BOp, 2, 16, 1>(taps, tu.zchunk, b, op, st);
-    case 4: return launch_batch_shape<BOp, 4, 8, 1>(taps, tu.zchunk, b, op, st);
-    case 5: return launch_batch_shape<BOp, 3, 8, 1>(taps, tu.zchunk, b, op, st);
-    case 6: return launch_batch_shape<BOp, 1, 4, 2>(taps, tu.zchunk, b, op, st);
-    case 7: return launch_batch_shape<BOp, 1, 4, 1>(taps, tu.zchunk, b, op, st);
-    case 8: return launch_batch_shape<BOp, 1, 8, 2>(taps, tu.zchunk, b, op, st);
-    case 9: return launch_batch_shape<BOp, 1, 8, 1>(taps, tu.zchunk, b, op, st);
-    case 10: return launch_batch_shape<BOp, 1, 12, 2>(taps, tu.zchunk, b, op, st);
-    case 11: return launch_batch_shape<BOp, 1, 16, 1>(taps, tu.zchunk, b, op, st);
+    case 0: return launch_batch_shape<BOp, 2, 4, 1, 1>(taps, tu.zchunk, b, op, st);
+    case 1: return launch_batch_shape<BOp, 4, 4, 1, 1>(taps, tu.zchunk, b, op, st);
+    case 2: return launch_batch_shape<BOp, 2, 8, 1, 1>(taps, tu.zchunk, b, op, st);
+    case 3: return launch_batch_shape<BOp, 2, 16, 1, 1>(taps, tu.zchunk, b, op, st);
+    case 4: return launch_batch_shape<BOp, 1, 8, 1, 1>(taps, tu.zchunk, b, op, st);
+    case 5: return launch_batch_shape<BOp, 1, 16, 1, 1>(taps, tu.zchunk, b, op, st);
+    case 6: return launch_batch_shape<BOp, 2, 4, 1, 2>(taps, tu.zchunk, b, op, st);
+    case 7: return launch_batch_shape<BOp, 1, 4, 1, 2>(taps, tu.zchunk, b, op, st);
+    case 8: return launch_batch_shape<BOp, 2, 8, 1, 2>(taps, tu.zchunk, b, op, st);
+    case 9: return launch_batch_shape<BOp, 1, 8, 1, 2>(taps, tu.zchunk, b, op, st);
+    case 10: return launch_batch_shape<BOp, 1, 12, 1, 2>(taps, tu.zchunk, b, op, st);
+    case 11: return launch_batch_shape<BOp, 1, 16, 1, 2>(taps, tu.zchunk, b, op, st);
     default: return hipErrorInvalidValue;
   }
 }

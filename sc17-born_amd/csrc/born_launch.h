@@ -15,9 +15,9 @@ struct Geometry {
 };
 
 // Tile shape: RY rows per thread, WY wavefronts per block, VX cells per lane
-// along x (tile 64*VX x RY*WY).  VX = 2 needs an even nx.
+// along x (tile 64*VX x RY*WY; VX = 2 needs an even nx), PZ planes per step.
 struct TileShape {
-  int ry, wy, vx;
+  int ry, wy, vx, pz;
 };
 constexpr int kNumShapes = 12;
 extern const TileShape kShapes[kNumShapes];
