@@ -31,6 +31,22 @@ def test_sinc_table_bit_exact_against_compiled_reference(jt):
         assert np.abs(oracle.sinc_table(jt) - ofast).max() < 1e-6
 
 
+@pytest.mark.parametrize("jt", range(1, 9))
+def test_sinc_table_matches_golden_vectors(jt):
+    """Golden vectors written by tests/golden/make_sinc_golden.py from the
+    compiled reference (bit patterns): the oracle equals the -O2 build bit for
+    bit and the -Ofast build within 1e-6."""
+    import json
+    import os
+
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "sinc_tables.json")))
+    ours = oracle.sinc_table(jt).ravel().view(np.uint32)
+    ref = np.array([int(h, 16) for h in g["tables"]["O2"][str(jt)]], np.uint32)
+    assert np.array_equal(ours, ref)
+    fast = np.array([int(h, 16) for h in g["tables"]["Ofast"][str(jt)]], np.uint32).view(np.float32)
+    assert np.abs(oracle.sinc_table(jt).ravel() - fast).max() < 1e-6
+
+
 def test_sinc_table_single_phase_is_a_delta():
     """jt=1: d=0, the Toeplitz right-hand side equals column 3, so the exact
     solution is e_3 (sinc_bob.cc:36-64)."""
