@@ -76,9 +76,29 @@ class Problem:
     receivers, sinc table and a domain-size model image."""
 
 
+def gaussian_anomalies(shape, d, count, amp, radius, seed, window=None):
+    """count Gaussian velocity anomalies exp(-(r/radius)^2) of +-amp m/s
+    (radius in cells), centres and signs seeded; window=(x0, x1, y0, y1,
+    z0, z1) restricts the centres to a sub-box of the grid."""
+    rng = np.random.default_rng(seed)
+    mz, my, mx = shape
+    x0, x1, y0, y1, z0, z1 = window or (0, mx, 0, my, 0, mz)
+    z = np.arange(mz, dtype=np.float32)[:, None, None]
+    y = np.arange(my, dtype=np.float32)[None, :, None]
+    x = np.arange(mx, dtype=np.float32)[None, None, :]
+    dv = np.zeros(shape, np.float32)
+    for _ in range(count):
+        cx, cy, cz = rng.uniform(x0, x1), rng.uniform(y0, y1), rng.uniform(z0, z1)
+        sign = 1.0 if rng.random() < 0.5 else -1.0
+        r2 = ((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2) / np.float32(radius * radius)
+        dv += np.float32(sign * amp) * np.exp(-r2)
+    return dv
+
+
 def make_problem(n=(256, 256, 256), d=10.0, nbound=20, nt=1500, dt=0.001, nshots=3,
                  rec=(128, 128), rec_d=20.0, rec_o=10.0, rec_depth=10.0, src_depth=20.0,
-                 fpeak=12.0, delay=0.1, nw=200, seed=1002, model_seed=None, src_xy=None):
+                 fpeak=12.0, delay=0.1, nw=200, seed=1002, model_seed=None, src_xy=None,
+                 anomalies=None):
     H = host()
     rng = np.random.default_rng(seed)
     p = Problem()
@@ -92,6 +112,8 @@ def make_problem(n=(256, 256, 256), d=10.0, nbound=20, nt=1500, dt=0.001, nshots
     p.o = -pad * d
     vz = velocity_law(mz, d)
     vel = np.ascontiguousarray(np.broadcast_to(vz[:, None, None], (mz, my, mx)), np.float32)
+    if anomalies is not None:
+        vel = (vel + gaussian_anomalies((mz, my, mx), d, **anomalies)).astype(np.float32)
     p.vmax = float(vel.max())
     # vnone: the propagation velocity of the Born fields (rtm_zero_op_3d.cc:157)
     p.V = np.zeros(p.nx * p.ny * p.nz, np.float32)
@@ -208,6 +230,14 @@ CONFIGS = {
     "c2": dict(n=(256, 256, 256), nt=1500, nshots=3, rec=(128, 128), rec_d=20.0,
                rec_o=10.0, seed=1002),
 }
+
+# FP16 error study: the 512x512x400 grid's velocity (v(z) plus 4 seeded
+# Gaussian anomalies of +-300 m/s, radius 40 cells), 2500 steps at 0.8 ms,
+# receivers every 20 m, restricted to a 128^3 crop around the shot (the FP64
+# oracle's share); the anomaly centres are drawn inside the crop.
+C3_CROP = dict(n=(128, 128, 128), nt=2500, dt=0.0008, nshots=1, rec=(64, 64), rec_d=20.0,
+               rec_o=10.0, seed=1003, src_xy=[(635.0, 635.0)],
+               anomalies=dict(count=4, amp=300.0, radius=40.0, seed=1003))
 
 # Batched shots: 32 windows of [200, 320]^2 x 300 cells at 10 m, 2000 steps
 BATCHES = {
