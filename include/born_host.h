@@ -106,6 +106,8 @@ typedef struct {
   int mode;            /* adjoint: 0 reference RTM, 1 exact transpose */
   int scatter;         /* 0: 1e-6 s m (reference), 1: dV B L(s) */
   int device;
+  int storage;         /* wavefield storage of the exact pair: 0 FP32, 1 FP16
+                          (born_set_storage; no reference counterpart) */
 } born_op_desc;
 
 /* rtm_zero_op ctor (rtm_zero_op_3d.cc:9-44) on a HIP device */

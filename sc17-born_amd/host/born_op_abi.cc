@@ -1,10 +1,12 @@
 // C ABI of the Born operator layer (include/born_host.h, part 2).
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <exception>
 #include <memory>
 #include <string>
 
+#include "born_hip.h"
 #include "born_host.h"
 #include "born_op.h"
 
@@ -60,6 +62,8 @@ int born_op_create(born_op **out, const born_op_desc *d) {
     s.mode = d->mode;
     s.scatter = d->scatter;
     h->prop = std::make_shared<hipProp>(d->device);
+    if (born_set_storage(h->prop->context(), d->storage) != BORN_OK)
+      throw std::runtime_error("storage must be 0 (FP32) or 1 (FP16)");
     h->op = std::make_shared<rtm_zero_op>(h->prop, s);
   });
   if (rc) {
@@ -124,6 +128,7 @@ int born_op_lsrtm(born_op *h, const float *data, float *model, int niter, double
     auto dv = wrap(h->op->data_space(), data);
     cgstep cg;
     simple_solver solver(&cg, dv, h->op);
+    solver.verb = getenv("BORN_VERBOSE") != nullptr;  // one line per iteration
     solver.solve(niter);
     auto m = std::dynamic_pointer_cast<hypercube_float>(solver.m);
     std::memcpy(model, m->vals.data(), m->vals.size() * sizeof(float));

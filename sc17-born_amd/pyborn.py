@@ -298,7 +298,8 @@ class OpDesc(C.Structure):
                 ("t0", C.c_float), ("dt", C.c_float), ("rec_nx", C.c_int), ("rec_ny", C.c_int),
                 ("rec_ox", C.c_float), ("rec_oy", C.c_float), ("rec_dx", C.c_float),
                 ("rec_dy", C.c_float), ("rec_depth", C.c_float), ("nbound", C.c_int),
-                ("mode", C.c_int), ("scatter", C.c_int), ("device", C.c_int)]
+                ("mode", C.c_int), ("scatter", C.c_int), ("device", C.c_int),
+                ("storage", C.c_int)]
 
 
 OP_SIGNATURES = {
@@ -334,7 +335,8 @@ class BornOp:
     mode 0: the reference's RTM adjoint; mode 1: the exact transpose."""
 
     def __init__(self, vel, vo, vd, wavelet, wdt, src_x, src_y, src_depth, nt, t0, dt,
-                 rec_n, rec_o, rec_d, rec_depth, nbound=50, mode=0, scatter=0, device=0):
+                 rec_n, rec_o, rec_d, rec_depth, nbound=50, mode=0, scatter=0, device=0,
+                 storage="f32"):
         self.lib = load_host_library()
         vel = np.ascontiguousarray(vel, np.float32)
         w = np.ascontiguousarray(wavelet, np.float32)
@@ -353,6 +355,7 @@ class BornOp:
         d.rec_ox, d.rec_oy = rec_o
         d.rec_dx, d.rec_dy = rec_d
         d.rec_depth, d.nbound, d.mode, d.scatter, d.device = rec_depth, nbound, mode, scatter, device
+        d.storage = Propagator.STORAGE[storage]
         h = C.c_void_p()
         rc = self.lib.born_op_create(C.byref(h), C.byref(d))
         self.h = h
