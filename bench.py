@@ -34,7 +34,7 @@ METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs F
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of this script
 # (profiles/born_fwd_adj_304cube_hbm_counters.md); reported as roofline.traffic
 # only when the measured workload and kernel are the profiled ones.
-PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 2, 8, 1, 1", "f32"): 1098.4e6}
+PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 2, 8, 1, 1, 64", "f32"): 1098.4e6}
 
 
 def load(name):
@@ -307,8 +307,9 @@ def main():
     achieved = fwd_alg_bytes / fwd_launch_s / 1e9
     alg_gbs = (FWD_BYTES_PER_CELL + ADJ_BYTES_PER_CELL) * cell_steps / elapsed / 1e9
     ft = tuning["fwd"]
-    fwd_kernel = "FwdOp<0, float>, %d, %d, 1, %d" % (ft["rows_per_thread"], ft["waves"],
-                                                     ft["planes_per_step"])
+    fwd_kernel = "FwdOp<0, float>, %d, %d, %d, %d, %d" % (
+        ft["rows_per_thread"], ft["waves"], ft["cells_per_lane"], ft["planes_per_step"],
+        64 // ft["cells_per_lane"])
     traffic = PROFILED_TRAFFIC.get((args.config, fwd_kernel, "f32"))
     if args.nt or args.shots:
         traffic = None

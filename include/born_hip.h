@@ -174,14 +174,16 @@ int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
                     int *waves_per_block, int *zchunk, float *ms);
 
 /* The chosen tile of a step kernel: tile_x x tile_y cells of the xy plane,
- * planes_per_step z planes advanced between two waits for memory. */
+ * cells_per_lane consecutive x cells per thread (2: one 8-byte / 4-byte load
+ * of a pair; needs an even nx), planes_per_step z planes advanced between two
+ * waits for memory. */
 int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y,
-                  int *planes_per_step);
+                  int *cells_per_lane, int *planes_per_step);
 /* Fix the launch shape of a step kernel (rows per thread x waves per block
- * x planes per step, one of the autotuner's shapes) and its z chunk; the
- * autotuner then leaves that kernel. */
+ * x cells per lane x planes per step, one of the autotuner's shapes) and its
+ * z chunk; the autotuner then leaves that kernel. */
 int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_per_block,
-                    int planes_per_step, int zchunk);
+                    int cells_per_lane, int planes_per_step, int zchunk);
 
 void *born_get_stream(const born_ctx *ctx);
 

@@ -1253,22 +1253,24 @@ int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
 }
 
 int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y,
-                  int *planes_per_step) {
+                  int *cells_per_lane, int *planes_per_step) {
   if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
   const born::TileShape &ts = born::kShapes[tuning(ctx, kernel).shape];
-  if (tile_x) *tile_x = born::kTX * ts.vx;
-  if (tile_y) *tile_y = ts.ry * ts.wy;
+  if (tile_x) *tile_x = ts.lx * ts.vx;
+  if (tile_y) *tile_y = ts.ry * (born::kTX * ts.wy / ts.lx);
+  if (cells_per_lane) *cells_per_lane = ts.vx;
   if (planes_per_step) *planes_per_step = ts.pz;
   return BORN_OK;
 }
 
 int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_per_block,
-                    int planes_per_step, int zchunk) {
+                    int cells_per_lane, int planes_per_step, int zchunk) {
   if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
   if (zchunk < 1) return fail(ctx, BORN_ERR_ARG, "zchunk must be positive");
   for (int sh = 0; sh < born::kNumShapes; sh++) {
     const born::TileShape &ts = born::kShapes[sh];
-    if (ts.ry == rows_per_thread && ts.wy == waves_per_block && ts.pz == planes_per_step) {
+    if (ts.ry == rows_per_thread && ts.wy == waves_per_block && ts.vx == cells_per_lane &&
+        ts.pz == planes_per_step) {
       ctx->tune[kernel] = {sh, zchunk};
       ctx->tune_ms[kernel] = 0.f;  // fixed: the autotuner leaves it
       return BORN_OK;
@@ -1625,8 +1627,9 @@ int born_batch_get_tile(const born_batch *b, int scatter, int *tile_x, int *tile
   if (!b || (scatter != 0 && scatter != 1)) return BORN_ERR_ARG;
   const int kind = (b->ctx->storage == BORN_STORAGE_F16 ? 2 : 0) + scatter;
   const born::Tune t = b->tune[kind];
-  if (tile_x) *tile_x = born::kTX * born::kShapes[t.shape].vx;
-  if (tile_y) *tile_y = born::kShapes[t.shape].ry * born::kShapes[t.shape].wy;
+  const born::TileShape &ts = born::kShapes[t.shape];
+  if (tile_x) *tile_x = ts.lx * ts.vx;
+  if (tile_y) *tile_y = ts.ry * (born::kTX * ts.wy / ts.lx);
   if (zchunk) *zchunk = t.zchunk;
   if (ms) *ms = b->tune_ms[kind];
   return BORN_OK;

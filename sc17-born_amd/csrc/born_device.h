@@ -249,14 +249,17 @@ __device__ __forceinline__ void st_vec(__half *p, uint32_t o, const float (&v)[V
 //             float (&tmax)[NF]);                the new values of one cell
 //   storev<VX>(const Blk&, uint32_t o, const float (&out)[NOUT][VX])
 //   void publish_max(int f, float m);           (NMAX > 0) block max of |new|
-template <class Op, int RY, int WY, int VX, int PZ>
+template <class Op, int RY, int WY, int VX, int PZ, int LX>
 __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const Sponge &sp,
                                            const Op &op, const int L) {
   constexpr int NF = Op::NF;
   constexpr int NP = Op::NPRE > 0 ? Op::NPRE : 1;
   constexpr int NO = Op::NOUT;
-  constexpr int TW = kTX * VX;
-  constexpr int TY = RY * WY;
+  // LX lanes per row (64: a wavefront per row; 32: two rows per wavefront),
+  // NG row groups per block
+  constexpr int NG = kTX * WY / LX;
+  constexpr int TW = LX * VX;
+  constexpr int TY = RY * NG;
   constexpr int LW = TW + 2 * kH;
   constexpr int LH = TY + 2 * kH;
   constexpr int NQ = 2 * kH + PZ;  // queue: planes z - kH .. z + kH + PZ - 1
@@ -266,8 +269,8 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   const int rest = L / g.ntx;
   const int by = rest % g.nty;
   const int bz = rest / g.nty;
-  const int tx = (int)threadIdx.x % kTX;
-  const int ty = (int)threadIdx.x / kTX;
+  const int tx = (int)threadIdx.x % LX;
+  const int ty = (int)threadIdx.x / LX;
   // x tiles start at the row start so that a wavefront's row maps onto whole
   // cache lines; the 4-cell shell at either end is masked
   const int x0 = bx * TW, y0 = kH + by * TY;
@@ -288,7 +291,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   float bxy[RY][VX];
 #pragma unroll
   for (int j = 0; j < RY; j++) {
-    yy[j] = y0 + ty + WY * j;
+    yy[j] = y0 + ty + NG * j;
     in[j] = xin && yy[j] < g.ny - kH;
     const int yc = min(yy[j], g.ny - 1);
     row[j] = (uint32_t)yc * g.nx + xc;
@@ -299,7 +302,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   // LDS halo in units of VX cells: 8 y-halo rows of 64 units and, per row,
   // kH/VX units on either side, spread over the block's threads
   constexpr int XU = kH / VX;
-  constexpr int NYH = 2 * kH * kTX;
+  constexpr int NYH = 2 * kH * LX;
   constexpr int NXH = 2 * XU * TY;
   constexpr int NHALO = NYH + NXH;
   constexpr int HPT = (NHALO + kTX * WY - 1) / (kTX * WY);
@@ -312,9 +315,9 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     hv[k] = h < NHALO;
     int lr, lc;
     if (h < NYH) {  // rows 0..3 and TY+4..TY+7
-      const int r = h / kTX;
+      const int r = h / LX;
       lr = r < kH ? r : TY + r;
-      lc = kH + VX * (h % kTX);
+      lc = kH + VX * (h % LX);
     } else {  // columns 0..3 and TW+4..TW+7 of rows 4..TY+3
       const int hh = min(h - NYH, NXH - 1);
       const int c = hh % (2 * XU);
@@ -413,7 +416,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
           for (int v = 0; v < VX; v++) q[f][j][v][2 * kH + p] = lead[p][f][j][v];
 #pragma unroll
         for (int p = 0; p < PZ; p++) {
-          float *d = &tile[p][f][kH + ty + WY * j][kH + VX * tx];
+          float *d = &tile[p][f][kH + ty + NG * j][kH + VX * tx];
           if constexpr (VX == 1)
             d[0] = q[f][j][0][kH + p];
           else
@@ -472,7 +475,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
 #pragma unroll
       for (int j = 0; j < RY; j++) {
         if (!in[j]) continue;
-        const int lr = kH + ty + WY * j, lc = kH + VX * tx;
+        const int lr = kH + ty + NG * j, lc = kH + VX * tx;
         float lap[VX][NF], ctr[VX][NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
@@ -562,7 +565,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     for (int f = 0; f < NF; f++) {
       float m = tmax[f];
       for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-      if (tx == 0) red[f][ty] = m;
+      if (threadIdx.x % kTX == 0) red[f][threadIdx.x / kTX] = m;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -576,10 +579,11 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   }
 }
 
-template <class Op, int RY, int WY, int VX, int PZ>
+template <class Op, int RY, int WY, int VX, int PZ, int LX>
 __global__ void __launch_bounds__(kTX *WY)
     march_kernel(const Grid g, const Taps t, const Sponge sp, const Op op) {
-  march_tile<Op, RY, WY, VX, PZ>(g, t, sp, op, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+  march_tile<Op, RY, WY, VX, PZ, LX>(g, t, sp, op,
+                                     xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // Tiles of a batch of shots with their own grids: shot s owns the logical
@@ -608,14 +612,14 @@ __host__ __device__ inline Grid tile_grid(int nx, int ny, int nz, int tw, int ty
 // One launch advances every shot of a batch by one step.  BOp::shot(s, tw,
 // ty, zc) returns {Grid, Sponge, Op} of shot s with the op's pointers moved to
 // the shot's part of the batch arrays.
-template <class BOp, int RY, int WY, int VX, int PZ>
+template <class BOp, int RY, int WY, int VX, int PZ, int LX>
 __global__ void __launch_bounds__(kTX *WY)
     march_batch_kernel(const BatchTiles bt, const Taps t, const BOp bop) {
   const int Lg = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   int s = 0;
   while (s + 1 < bt.n && Lg >= bt.tile0[s + 1]) s++;
-  const auto sh = bop.shot(s, kTX * VX, RY * WY, bt.zc);
-  march_tile<typename BOp::Op, RY, WY, VX, PZ>(sh.g, t, sh.sp, sh.op, Lg - bt.tile0[s]);
+  const auto sh = bop.shot(s, LX * VX, RY * (kTX * WY / LX), bt.zc);
+  march_tile<typename BOp::Op, RY, WY, VX, PZ, LX>(sh.g, t, sh.sp, sh.op, Lg - bt.tile0[s]);
 }
 
 }  // namespace born

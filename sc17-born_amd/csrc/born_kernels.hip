@@ -162,12 +162,16 @@ struct FwdOp {
       tmax[1] = fmaxf(tmax[1], fabsf(rn));
     }
     if (pl.slot >= 0 && extract) {
+      // the 8 samples of a receiver are distinct, but the compiler cannot
+      // tell; all loads first, then all stores, one wait instead of eight
       for (int i = rec_first(rm, pl.slot, c.x, c.y, nx, ny); i >= 0; i = rm.rnext[i]) {
         float *q = tr + (long long)id * rm.nrec + i;
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = q[(long long)k * rm.nrec];
 #pragma unroll
         for (int k = 0; k < 8; k++)
-          q[(long long)k * rm.nrec] =
-              __fadd_rn(q[(long long)k * rm.nrec], __fmul_rn(rn, tab[k]));
+          q[(long long)k * rm.nrec] = __fadd_rn(v[k], __fmul_rn(rn, ldc(tab, k)));
       }
     }
   }
@@ -571,38 +575,40 @@ __global__ void transpose_kernel(float *dst, const float *src, int n0, int n1) {
 // ------------------------------------------------------------- launchers
 
 const TileShape kShapes[kNumShapes] = {
-    {2, 4, 1, 1}, {4, 4, 1, 1}, {2, 8, 1, 1}, {2, 16, 1, 1}, {1, 8, 1, 1}, {1, 16, 1, 1},
-    {2, 4, 1, 2}, {1, 4, 1, 2}, {2, 8, 1, 2}, {1, 8, 1, 2}, {1, 12, 1, 2}, {1, 16, 1, 2}};
+    {2, 4, 1, 1, 64}, {4, 4, 1, 1, 64}, {2, 8, 1, 1, 64}, {2, 16, 1, 1, 64},
+    {1, 8, 1, 1, 64}, {1, 16, 1, 1, 64}, {1, 4, 2, 1, 32}, {2, 4, 2, 1, 32},
+    {1, 8, 2, 1, 32}, {2, 8, 2, 1, 32}, {1, 12, 2, 1, 32}, {2, 2, 2, 1, 32}};
 
 static Grid make_grid(const Geometry &geo, const TileShape &ts, int zc) {
-  return tile_grid(geo.nx, geo.ny, geo.nz, kTX * ts.vx, ts.ry * ts.wy, zc > 0 ? zc : 32);
+  return tile_grid(geo.nx, geo.ny, geo.nz, ts.lx * ts.vx, ts.ry * (kTX * ts.wy / ts.lx),
+                   zc > 0 ? zc : 32);
 }
 
-template <class Op, int RY, int WY, int VX, int PZ>
+template <class Op, int RY, int WY, int VX, int PZ, int LX>
 static hipError_t launch_shape(const Geometry &geo, int zc, const Op &op, hipStream_t st) {
   if (VX == 2 && geo.nx % 2) return hipErrorInvalidValue;
-  const Grid g = make_grid(geo, {RY, WY, VX, PZ}, zc);
+  const Grid g = make_grid(geo, {RY, WY, VX, PZ, LX}, zc);
   const int nb = g.ntx * g.nty * g.ntz;
-  hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX, PZ>), dim3(nb), dim3(kTX * WY), 0, st, g,
-                     geo.taps, geo.sponge, op);
+  hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX, PZ, LX>), dim3(nb), dim3(kTX * WY), 0, st,
+                     g, geo.taps, geo.sponge, op);
   return hipGetLastError();
 }
 
 template <class Op>
 static hipError_t launch(const Geometry &geo, const Tune &tu, const Op &op, hipStream_t st) {
   switch (tu.shape) {
-    case 0: return launch_shape<Op, 2, 4, 1, 1>(geo, tu.zchunk, op, st);
-    case 1: return launch_shape<Op, 4, 4, 1, 1>(geo, tu.zchunk, op, st);
-    case 2: return launch_shape<Op, 2, 8, 1, 1>(geo, tu.zchunk, op, st);
-    case 3: return launch_shape<Op, 2, 16, 1, 1>(geo, tu.zchunk, op, st);
-    case 4: return launch_shape<Op, 1, 8, 1, 1>(geo, tu.zchunk, op, st);
-    case 5: return launch_shape<Op, 1, 16, 1, 1>(geo, tu.zchunk, op, st);
-    case 6: return launch_shape<Op, 2, 4, 1, 2>(geo, tu.zchunk, op, st);
-    case 7: return launch_shape<Op, 1, 4, 1, 2>(geo, tu.zchunk, op, st);
-    case 8: return launch_shape<Op, 2, 8, 1, 2>(geo, tu.zchunk, op, st);
-    case 9: return launch_shape<Op, 1, 8, 1, 2>(geo, tu.zchunk, op, st);
-    case 10: return launch_shape<Op, 1, 12, 1, 2>(geo, tu.zchunk, op, st);
-    case 11: return launch_shape<Op, 1, 16, 1, 2>(geo, tu.zchunk, op, st);
+    case 0: return launch_shape<Op, 2, 4, 1, 1, 64>(geo, tu.zchunk, op, st);
+    case 1: return launch_shape<Op, 4, 4, 1, 1, 64>(geo, tu.zchunk, op, st);
+    case 2: return launch_shape<Op, 2, 8, 1, 1, 64>(geo, tu.zchunk, op, st);
+    case 3: return launch_shape<Op, 2, 16, 1, 1, 64>(geo, tu.zchunk, op, st);
+    case 4: return launch_shape<Op, 1, 8, 1, 1, 64>(geo, tu.zchunk, op, st);
+    case 5: return launch_shape<Op, 1, 16, 1, 1, 64>(geo, tu.zchunk, op, st);
+    case 6: return launch_shape<Op, 1, 4, 2, 1, 32>(geo, tu.zchunk, op, st);
+    case 7: return launch_shape<Op, 2, 4, 2, 1, 32>(geo, tu.zchunk, op, st);
+    case 8: return launch_shape<Op, 1, 8, 2, 1, 32>(geo, tu.zchunk, op, st);
+    case 9: return launch_shape<Op, 2, 8, 2, 1, 32>(geo, tu.zchunk, op, st);
+    case 10: return launch_shape<Op, 1, 12, 2, 1, 32>(geo, tu.zchunk, op, st);
+    case 11: return launch_shape<Op, 2, 2, 2, 1, 32>(geo, tu.zchunk, op, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -737,7 +743,7 @@ hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
   return hipGetLastError();
 }
 
-template <class BOp, int RY, int WY, int VX, int PZ>
+template <class BOp, int RY, int WY, int VX, int PZ, int LX>
 static hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdStep &b,
                                      const BOp &op, hipStream_t st) {
   if (b.nshots < 1 || b.nshots > kMaxBatch) return hipErrorInvalidValue;
@@ -748,13 +754,13 @@ static hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdSte
   for (int s = 0; s < b.nshots; s++) {
     const ShotDesc &d = b.host_desc[s];
     if (VX == 2 && d.nx % 2) return hipErrorInvalidValue;
-    const Grid g = tile_grid(d.nx, d.ny, d.nz, kTX * VX, RY * WY, bt.zc);
+    const Grid g = tile_grid(d.nx, d.ny, d.nz, LX * VX, RY * (kTX * WY / LX), bt.zc);
     bt.tile0[s] = tiles;
     tiles += g.ntx * g.nty * g.ntz;
   }
   bt.tile0[b.nshots] = tiles;
-  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX, PZ>), dim3(tiles), dim3(kTX * WY),
-                     0, st, bt, taps, op);
+  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX, PZ, LX>), dim3(tiles),
+                     dim3(kTX * WY), 0, st, bt, taps, op);
   return hipGetLastError();
 }
 
@@ -762,18 +768,18 @@ template <class BOp>
 static hipError_t launch_batch(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
                                const BOp &op, hipStream_t st) {
   switch (tu.shape) {
-    case 0: return launch_batch_shape<BOp, 2, 4, 1, 1>(taps, tu.zchunk, b, op, st);
-    case 1: return launch_batch_shape<BOp, 4, 4, 1, 1>(taps, tu.zchunk, b, op, st);
-    case 2: return launch_batch_shape<BOp, 2, 8, 1, 1>(taps, tu.zchunk, b, op, st);
-    case 3: return launch_batch_shape<BOp, 2, 16, 1, 1>(taps, tu.zchunk, b, op, st);
-    case 4: return launch_batch_shape<BOp, 1, 8, 1, 1>(taps, tu.zchunk, b, op, st);
-    case 5: return launch_batch_shape<BOp, 1, 16, 1, 1>(taps, tu.zchunk, b, op, st);
-    case 6: return launch_batch_shape<BOp, 2, 4, 1, 2>(taps, tu.zchunk, b, op, st);
-    case 7: return launch_batch_shape<BOp, 1, 4, 1, 2>(taps, tu.zchunk, b, op, st);
-    case 8: return launch_batch_shape<BOp, 2, 8, 1, 2>(taps, tu.zchunk, b, op, st);
-    case 9: return launch_batch_shape<BOp, 1, 8, 1, 2>(taps, tu.zchunk, b, op, st);
-    case 10: return launch_batch_shape<BOp, 1, 12, 1, 2>(taps, tu.zchunk, b, op, st);
-    case 11: return launch_batch_shape<BOp, 1, 16, 1, 2>(taps, tu.zchunk, b, op, st);
+    case 0: return launch_batch_shape<BOp, 2, 4, 1, 1, 64>(taps, tu.zchunk, b, op, st);
+    case 1: return launch_batch_shape<BOp, 4, 4, 1, 1, 64>(taps, tu.zchunk, b, op, st);
+    case 2: return launch_batch_shape<BOp, 2, 8, 1, 1, 64>(taps, tu.zchunk, b, op, st);
+    case 3: return launch_batch_shape<BOp, 2, 16, 1, 1, 64>(taps, tu.zchunk, b, op, st);
+    case 4: return launch_batch_shape<BOp, 1, 8, 1, 1, 64>(taps, tu.zchunk, b, op, st);
+    case 5: return launch_batch_shape<BOp, 1, 16, 1, 1, 64>(taps, tu.zchunk, b, op, st);
+    case 6: return launch_batch_shape<BOp, 1, 4, 2, 1, 32>(taps, tu.zchunk, b, op, st);
+    case 7: return launch_batch_shape<BOp, 2, 4, 2, 1, 32>(taps, tu.zchunk, b, op, st);
+    case 8: return launch_batch_shape<BOp, 1, 8, 2, 1, 32>(taps, tu.zchunk, b, op, st);
+    case 9: return launch_batch_shape<BOp, 2, 8, 2, 1, 32>(taps, tu.zchunk, b, op, st);
+    case 10: return launch_batch_shape<BOp, 1, 12, 2, 1, 32>(taps, tu.zchunk, b, op, st);
+    case 11: return launch_batch_shape<BOp, 2, 2, 2, 1, 32>(taps, tu.zchunk, b, op, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -15,9 +15,11 @@ struct Geometry {
 };
 
 // Tile shape: RY rows per thread, WY wavefronts per block, VX cells per lane
-// along x (tile 64*VX x RY*WY; VX = 2 needs an even nx), PZ planes per step.
+// along x (VX = 2 needs an even nx), PZ planes per step, LX lanes per row
+// (64, or 32: a wavefront covers two rows).  The tile is LX*VX x
+// RY*(64*WY/LX) cells.
 struct TileShape {
-  int ry, wy, vx, pz;
+  int ry, wy, vx, pz, lx;
 };
 constexpr int kNumShapes = 12;
 extern const TileShape kShapes[kNumShapes];

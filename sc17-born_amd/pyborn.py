@@ -59,8 +59,9 @@ SIGNATURES = {
     "born_set_autotune": (_i, [_vp, _i]),
     "born_get_tuning": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
                              C.POINTER(C.c_float)]),
-    "born_get_tile": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
-    "born_set_tuning": (_i, [_vp, _i, _i, _i, _i, _i]),
+    "born_get_tile": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
+                           C.POINTER(_i)]),
+    "born_set_tuning": (_i, [_vp, _i, _i, _i, _i, _i, _i]),
     "born_get_stream": (_vp, [_vp]),
     "born_batch_create": (_i, [_vp, _i, _vp, _f, _f, _f, C.POINTER(_vp)]),
     "born_batch_destroy": (_i, [_vp]),
@@ -244,18 +245,20 @@ class Propagator:
             ry, wy, zc, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
             self._chk(self.lib.born_get_tuning(self.h, k, C.byref(ry), C.byref(wy), C.byref(zc),
                                                C.byref(ms)), "tuning")
-            tx, tyy, pz = C.c_int(), C.c_int(), C.c_int()
-            self._chk(self.lib.born_get_tile(self.h, k, C.byref(tx), C.byref(tyy), C.byref(pz)),
-                      "tile")
+            tx, tyy, vx, pz = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            self._chk(self.lib.born_get_tile(self.h, k, C.byref(tx), C.byref(tyy), C.byref(vx),
+                                             C.byref(pz)), "tile")
             out[name] = {"tile": [tx.value, tyy.value], "rows_per_thread": ry.value,
-                         "waves": wy.value, "planes_per_step": pz.value,
-                         "zchunk": zc.value, "ms": round(ms.value, 4)}
+                         "waves": wy.value, "cells_per_lane": vx.value,
+                         "planes_per_step": pz.value, "zchunk": zc.value,
+                         "ms": round(ms.value, 4)}
         return out
 
-    def set_tuning(self, kernel, rows_per_thread, waves, planes_per_step, zchunk):
+    def set_tuning(self, kernel, rows_per_thread, waves, cells_per_lane, planes_per_step,
+                   zchunk):
         k = self.KERNELS.index(kernel)
-        self._chk(self.lib.born_set_tuning(self.h, k, rows_per_thread, waves, planes_per_step,
-                                           zchunk), "set_tuning")
+        self._chk(self.lib.born_set_tuning(self.h, k, rows_per_thread, waves, cells_per_lane,
+                                           planes_per_step, zchunk), "set_tuning")
 
     @property
     def stream(self):
