@@ -158,6 +158,16 @@ int born_last_checkpoint_plan(const born_ctx *ctx, int *segments,
 #define BORN_STORAGE_F16 1
 int born_set_storage(born_ctx *ctx, int storage);
 
+/* Two time steps per launch in the Born forward with FP32 storage: one
+ * sweep reads levels k-1 and k once and writes levels k+1 and k+2; the
+ * values are those of single steps.  mode 0 (default): single steps;
+ * 1: pairs unless the autotuner timed two single steps faster; 2: pairs.
+ * No reference counterpart: the reference advances one step per pass
+ * (base/cpu_prop.cc:31-53). */
+int born_set_step_pairs(born_ctx *ctx, int mode);
+/* 1 when the last Born forward advanced in two-step launches */
+int born_last_step_pairs(const born_ctx *ctx);
+
 /* z planes per block of the step kernels (0 = chosen by the autotuner) */
 int born_set_zchunk(born_ctx *ctx, int zchunk);
 
@@ -168,8 +178,9 @@ int born_set_zchunk(born_ctx *ctx, int zchunk);
 int born_set_autotune(born_ctx *ctx, int on);
 /* The chosen launch shape of a step kernel: 0 Born forward, 1 Born forward
  * (dV scatter), 2 source propagation, 3 reference adjoint, 4/5 background
- * recompute, 6 adjoint state; 7-11 the FP16-storage forms of 0, 1, 4, 5, 6.
- * ms = timed duration (-1 when not tuned). */
+ * recompute, 6 adjoint state; 7-11 the FP16-storage forms of 0, 1, 4, 5, 6;
+ * 12, 13 the two-step forms of 0, 1 (tile 56 x rows*waves).
+ * ms = timed duration of one launch (-1 when not tuned). */
 int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
                     int *waves_per_block, int *zchunk, float *ms);
 

@@ -42,8 +42,9 @@ struct born_ctx {
   float *d_bx = nullptr, *d_by = nullptr, *d_bz = nullptr;
   int zchunk = 0;  // override of the tuned z chunk (0: tuned)
   int autotune = 1;
-  born::Tune tune[12];
-  float tune_ms[12];
+  born::Tune tune[14];
+  float tune_ms[14];
+  int pairs = 0;  // two-step forward launches for FP32 levels: 0 off, 1 auto, 2 on
 
   int nsinc_s = 0, jts = 0, nsinc_d = 0, jtd = 0;
   float *d_tab_s = nullptr, *d_tab_d = nullptr;
@@ -73,6 +74,7 @@ struct born_ctx {
   size_t ckring_cap = 0;
 
   float *d_w[4] = {nullptr, nullptr, nullptr, nullptr};
+  float *d_w2[4] = {nullptr, nullptr, nullptr, nullptr};  // levels k+1, k+2 of pairs
   float *d_img = nullptr, *d_acc = nullptr, *d_p0 = nullptr, *d_p1 = nullptr;
   float *d_tr = nullptr, *d_tr2 = nullptr;
   size_t tr_cap = 0;
@@ -88,6 +90,7 @@ struct born_ctx {
   long long hist_version = -1;
   int hist_scatter = -1, hist_jt = -1, hist_nt = -1, hist_storage = -1;
   int last_segments = 0, last_recomputed = 0;
+  int last_pairs = 0;
 };
 
 namespace {
@@ -177,15 +180,18 @@ born::RecMap recmap(const born_ctx *c) {
 // step kernels that are tuned separately
 enum Kind {
   K_FWD0, K_FWD1, K_SRC, K_RADJ, K_SREC0, K_SREC1, K_LAM,
-  K_FWD0H, K_FWD1H, K_SREC0H, K_SREC1H, K_LAMH, K_NUM
+  K_FWD0H, K_FWD1H, K_SREC0H, K_SREC1H, K_LAMH, K_FWD0P, K_FWD1P, K_NUM
 };
+static_assert(K_NUM <= 14, "tuning slots");
+bool is_pair(int kind) { return kind == K_FWD0P || kind == K_FWD1P; }
 
 born::SrcPts points(const born_ctx *c) { return {c->d_pts, c->npts_s}; }
 born::SrcPts no_points() { return {nullptr, 0}; }
 
 void reset_tuning(born_ctx *c) {
   for (int k = 0; k < K_NUM; k++) {
-    c->tune[k] = {3, 64};  // 64x32 tiles of 2 rows per thread, 64 planes
+    // 64x32 tiles of 2 rows per thread, 64 planes; pairs: 56x16, 128 planes
+    c->tune[k] = is_pair(k) ? born::Tune{0, 128} : born::Tune{3, 64};
     c->tune_ms[k] = -1.f;
   }
 }
@@ -235,6 +241,49 @@ int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx,
 template <class F>
 int autotune(born_ctx *ctx, int kind, F run) {
   return autotune_into(ctx, &ctx->tune[kind], &ctx->tune_ms[kind], ctx->nx % 2 == 0, run);
+}
+
+// The same for a two-step kernel: its shapes and z chunks that split the
+// interior evenly (a chunk recomputes 8 planes of its neighbours, so chunks
+// are long).  tune_ms is per launch (2 steps).
+template <class F>
+int autotune_pair(born_ctx *ctx, int kind, F run) {
+  float *tune_ms = &ctx->tune_ms[kind];
+  if (!ctx->autotune || *tune_ms >= 0.f) return BORN_OK;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  born::Tune bt = ctx->tune[kind];
+  // chunks of equal length: every block marches the same number of planes
+  int zcs[6], nzc = 0;
+  const int nzi = std::max(ctx->nz - 8, 1);
+  for (int k = 1; k <= 6 && nzc < 6; k++) {
+    const int zc = (nzi + k - 1) / k;
+    if (zc >= 16 && (nzc == 0 || zcs[nzc - 1] != zc)) zcs[nzc++] = zc;
+  }
+  if (nzc == 0) zcs[nzc++] = nzi;
+  for (int sh = 0; sh < born::kNumPairShapes; sh++)
+    for (int zi = 0; zi < nzc; zi++) {
+      const born::Tune t{sh, ctx->zchunk > 0 ? ctx->zchunk : zcs[zi]};
+      if (ctx->zchunk > 0 && zi > 0) continue;
+      HIPCHK(run(t));  // warm
+      HIPCHK(hipEventRecord(e0, ctx->stream));
+      for (int r = 0; r < 3; r++) HIPCHK(run(t));
+      HIPCHK(hipEventRecord(e1, ctx->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bt = t;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->tune[kind] = bt;
+  *tune_ms = best / 3.f;
+  return BORN_OK;
 }
 
 // Sponge factor per axis coordinate (cpu_prop.cc:141-165, 329-332).
@@ -290,6 +339,12 @@ int check_dims(born_ctx *ctx, int n1, int n2, int n3) {
 int ensure_work(born_ctx *ctx) {
   for (int i = 0; i < 4; i++)
     if (!ctx->d_w[i]) CHK(dev_alloc(ctx, &ctx->d_w[i], (size_t)ctx->n));
+  return BORN_OK;
+}
+
+int ensure_pair_work(born_ctx *ctx) {
+  for (int i = 0; i < 4; i++)
+    if (!ctx->d_w2[i]) CHK(dev_alloc(ctx, &ctx->d_w2[i], (size_t)ctx->n));
   return BORN_OK;
 }
 
@@ -559,7 +614,8 @@ int born_destroy(born_ctx *ctx) {
   void *bufs[] = {ctx->d_pts,   ctx->d_bx,    ctx->d_by,    ctx->d_bz,   ctx->d_tab_s,
                   ctx->d_tab_d, ctx->d_src,   ctx->d_vel1, ctx->d_vel2,
                   ctx->d_plane_slot, ctx->d_rmap, ctx->d_rnext, ctx->d_w[0],
-                  ctx->d_w[1],  ctx->d_w[2],  ctx->d_w[3], ctx->d_img,
+                  ctx->d_w[1],  ctx->d_w[2],  ctx->d_w[3], ctx->d_w2[0],
+                  ctx->d_w2[1], ctx->d_w2[2], ctx->d_w2[3], ctx->d_img,
                   ctx->d_acc,   ctx->d_p0,    ctx->d_p1,   ctx->d_tr,
                   ctx->d_tr2,   ctx->d_amp,   ctx->d_seg,  ctx->d_ckpt};
   for (void *b : bufs)
@@ -764,6 +820,8 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   const long long n = ctx->n;
   const bool half = ctx->storage == BORN_STORAGE_F16;
   const size_t esz = elem(ctx);
+  const bool try_pairs = ctx->pairs && !half && nt >= 2;
+  if (try_pairs) CHK(ensure_pair_work(ctx));
   float *hist = nullptr;
   if (keep_background && nt > 0) {
     if ((size_t)nt * n * esz <= history_budget(ctx)) {
@@ -791,6 +849,22 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
       return born::launch_fwd_step(geo, tu, t, ctx->stream); }));
   }
   const born::Tune tu = tuning(ctx, kind);
+  const int pkind = scatter == 0 ? K_FWD0P : K_FWD1P;
+  bool use_pairs = false;
+  if (try_pairs) {
+    born::PairStep t{};
+    t.s_old = ctx->d_w[0]; t.s_cur = ctx->d_w[1]; t.r_old = ctx->d_w[2]; t.r_cur = ctx->d_w[3];
+    t.s_a = ctx->d_w2[0]; t.s_b = ctx->d_w2[1]; t.r_a = ctx->d_w2[2]; t.r_b = ctx->d_w2[3];
+    t.vel = ctx->d_vel2; t.model = model; t.src = no_points(); t.rec = recmap(ctx);
+    t.tr = traces; t.tab_a = t.tab_b = ctx->d_tab_d; t.scatter = scatter;
+    CHK(autotune_pair(ctx, pkind, [&](const born::Tune &tp) {
+      return born::launch_pair_step(geo, tp, t, ctx->stream); }));
+    // pairs unless both kernels were timed and two single steps won
+    const float mp = ctx->tune_ms[pkind], ms = ctx->tune_ms[kind];
+    use_pairs = ctx->pairs == 2 || !(mp > 0.f && ms > 0.f && mp > 2.f * ms);
+    for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w2[i], n));
+  }
+  ctx->last_pairs = use_pairs;
   for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w[i], n));
   CHK(source_amps(ctx, nt, jt, 0, 1 / (float)jt));
   if (half) {
@@ -799,8 +873,42 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     HIPCHK(born::launch_fill(ctx->d_ring + MAX_M, 1, 0.f, ctx->stream));
     HIPCHK(born::launch_absmax(model, n, (unsigned *)(ctx->d_ring + MAX_M), ctx->stream));
   }
-  float *s_old = ctx->d_w[0], *s_cur = ctx->d_w[1];
-  float *r_old = ctx->d_w[2], *r_cur = ctx->d_w[3];
+  int it = 0;
+  // levels k-1, k, k+1, k+2 of s and r rotate through four buffers each
+  float *sb[4] = {ctx->d_w[0], ctx->d_w[1], ctx->d_w2[0], ctx->d_w2[1]};
+  float *rb[4] = {ctx->d_w[2], ctx->d_w[3], ctx->d_w2[2], ctx->d_w2[3]};
+  if (use_pairs) {
+    const born::Tune tp = tuning(ctx, pkind);
+    born::PairStep p{};
+    p.vel = ctx->d_vel2;
+    p.model = model;
+    p.src = points(ctx);
+    p.rec = recmap(ctx);
+    p.tr = traces;
+    p.scatter = scatter;
+    for (; it + 1 < nt; it += 2) {
+      const int ida = it / jt, idb = (it + 1) / jt;
+      p.s_old = sb[0]; p.s_cur = sb[1]; p.s_a = sb[2]; p.s_b = sb[3];
+      p.r_old = rb[0]; p.r_cur = rb[1]; p.r_a = rb[2]; p.r_b = rb[3];
+      p.h_a = hist ? nth(hist, it, n, esz) : nullptr;
+      p.h_b = hist ? nth(hist, it + 1, n, esz) : nullptr;
+      p.amp_a = ctx->d_amp + (long long)it * ctx->npts_s;
+      p.amp_b = ctx->d_amp + (long long)(it + 1) * ctx->npts_s;
+      p.tab_a = ctx->d_tab_d + (long long)(it - ida * jt) * 8;
+      p.tab_b = ctx->d_tab_d + (long long)(it + 1 - idb * jt) * 8;
+      p.id_a = ida;
+      p.id_b = idb;
+      p.ext_a = ida + 7 < ctx->nt_rec;
+      p.ext_b = idb + 7 < ctx->nt_rec;
+      HIPCHK(born::launch_pair_step(geo, tp, p, ctx->stream));
+      std::swap(sb[0], sb[2]);
+      std::swap(sb[1], sb[3]);
+      std::swap(rb[0], rb[2]);
+      std::swap(rb[1], rb[3]);
+    }
+  }
+  float *s_old = sb[0], *s_cur = sb[1];
+  float *r_old = rb[0], *r_cur = rb[1];
   born::FwdStep a{};
   a.vel = ctx->d_vel2;
   a.model = model;
@@ -811,13 +919,26 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   a.ring[0] = ring(ctx, RING_S);
   a.ring[1] = ring(ctx, RING_R);
   a.mmax = maxp(ctx, MAX_M);
-  for (int it = 0; it < nt; it++) {
+  // With scatter 0 the scatter operand of step it is s of level it+1, so a
+  // kept history holds the s levels: step it reads levels it (history it-1)
+  // and it-1 (history it-2) and writes level it+1 into history it, one
+  // write per cell less than a separate s buffer and history.
+  const bool s_in_hist = hist && scatter == 0;
+  for (; it < nt; it++) {
     const int id = it / jt, ii = it - id * jt;
-    a.s1 = s_cur;
-    a.s0 = s_old;
+    if (s_in_hist) {
+      a.s1 = it >= 1 ? nth(hist, it - 1, n, esz) : s_cur;  // level 0: zero
+      a.s0 = it >= 2 ? nth(hist, it - 2, n, esz) : s_old;
+      a.s_new = nth(hist, it, n, esz);
+      a.hist = nullptr;
+    } else {
+      a.s1 = s_cur;
+      a.s0 = s_old;
+      a.s_new = nullptr;
+      a.hist = hist ? nth(hist, it, n, esz) : nullptr;
+    }
     a.r1 = r_cur;
     a.r0 = r_old;
-    a.hist = hist ? nth(hist, it, n, esz) : nullptr;
     a.hscale = half && hist ? ctx->d_hscale + it : nullptr;
     a.amp = ctx->d_amp + (long long)it * ctx->npts_s;
     a.tab = ctx->d_tab_d + (long long)ii * 8;
@@ -1225,6 +1346,14 @@ int born_set_zchunk(born_ctx *ctx, int zchunk) {
   return BORN_OK;
 }
 
+int born_set_step_pairs(born_ctx *ctx, int mode) {
+  if (!ctx || mode < 0 || mode > 2) return BORN_ERR_ARG;
+  ctx->pairs = mode;
+  return BORN_OK;
+}
+
+int born_last_step_pairs(const born_ctx *ctx) { return ctx ? ctx->last_pairs : 0; }
+
 int born_set_storage(born_ctx *ctx, int storage) {
   if (!ctx) return BORN_ERR_ARG;
   if (storage != BORN_STORAGE_F32 && storage != BORN_STORAGE_F16)
@@ -1245,8 +1374,10 @@ int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
                     int *waves_per_block, int *zchunk, float *ms) {
   if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
   const born::Tune t = tuning(ctx, kernel);
-  if (rows_per_thread) *rows_per_thread = born::kShapes[t.shape].ry;
-  if (waves_per_block) *waves_per_block = born::kShapes[t.shape].wy;
+  if (rows_per_thread)
+    *rows_per_thread = is_pair(kernel) ? born::kPairShapes[t.shape].ry : born::kShapes[t.shape].ry;
+  if (waves_per_block)
+    *waves_per_block = is_pair(kernel) ? born::kPairShapes[t.shape].wy : born::kShapes[t.shape].wy;
   if (zchunk) *zchunk = t.zchunk;
   if (ms) *ms = ctx->tune_ms[kernel];
   return BORN_OK;
@@ -1255,6 +1386,14 @@ int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
 int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y,
                   int *cells_per_lane, int *planes_per_step) {
   if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
+  if (is_pair(kernel)) {
+    const born::PairShape &ps = born::kPairShapes[tuning(ctx, kernel).shape];
+    if (tile_x) *tile_x = born::kTX - 2 * born::kH;
+    if (tile_y) *tile_y = ps.ry * ps.wy;
+    if (cells_per_lane) *cells_per_lane = 1;
+    if (planes_per_step) *planes_per_step = 1;
+    return BORN_OK;
+  }
   const born::TileShape &ts = born::kShapes[tuning(ctx, kernel).shape];
   if (tile_x) *tile_x = ts.lx * ts.vx;
   if (tile_y) *tile_y = ts.ry * (born::kTX * ts.wy / ts.lx);
@@ -1267,6 +1406,18 @@ int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_pe
                     int cells_per_lane, int planes_per_step, int zchunk) {
   if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
   if (zchunk < 1) return fail(ctx, BORN_ERR_ARG, "zchunk must be positive");
+  if (is_pair(kernel)) {
+    for (int sh = 0; sh < born::kNumPairShapes; sh++) {
+      const born::PairShape &ps = born::kPairShapes[sh];
+      if (ps.ry == rows_per_thread && ps.wy == waves_per_block && cells_per_lane == 1 &&
+          planes_per_step == 1) {
+        ctx->tune[kernel] = {sh, zchunk};
+        ctx->tune_ms[kernel] = 0.f;
+        return BORN_OK;
+      }
+    }
+    return fail(ctx, BORN_ERR_ARG, "no such launch shape");
+  }
   for (int sh = 0; sh < born::kNumShapes; sh++) {
     const born::TileShape &ts = born::kShapes[sh];
     if (ts.ry == rows_per_thread && ts.wy == waves_per_block && ts.vx == cells_per_lane &&

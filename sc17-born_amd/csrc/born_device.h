@@ -144,13 +144,41 @@ __device__ __forceinline__ int xcd_remap(int h, int nb) {
   return xcd * q + (xcd < r ? xcd : r) + k;
 }
 
+// Arithmetic with its rounding fixed.  The compiler fuses a product into a
+// following sum where it can; these products and sums are never fused, and
+// fma_r always is, so every kernel that computes a cell in the same terms
+// gets the same bits.
+__device__ __forceinline__ float add_r(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ float mul_r(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ double add_r(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ double mul_r(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// One leapfrog update b * (V lap + 2 cur - b old) (cpu_prop.cc:31-53 with
+// the damping of cpu_prop.cc:141-165), as fma(V, lap, cur) + cur - b old.
+__device__ __forceinline__ float leap(float b, float v, float lap, float cur, float old) {
+  return mul_r(b, fma_r(-b, old, add_r(fma_r(v, lap, cur), cur)));
+}
+
 // Eight-tap sinc interpolation with the reference's rounding: products and
 // sums rounded separately, left to right (base/cpu_prop.cc:169-176).
 __device__ __forceinline__ float sinc8(const float *t, const float *v,
                                        long long stride) {
-  float s = __fmul_rn(t[0], v[0]);
+  float s = mul_r(t[0], v[0]);
 #pragma unroll
-  for (int k = 1; k < 8; k++) s = __fadd_rn(s, __fmul_rn(t[k], v[k * stride]));
+  for (int k = 1; k < 8; k++) s = add_r(s, mul_r(t[k], v[k * stride]));
   return s;
 }
 
@@ -183,7 +211,7 @@ __device__ __forceinline__ float add_points(const Points &pts, uint32_t m, int x
   for (uint32_t b = m; b; b &= b - 1) {
     const int i = __builtin_ctz(b);
     const int4 q = ldc(pts.p, i);
-    if (q.x == x && q.y == y) v = __fadd_rn(v, ldc(amp, i));
+    if (q.x == x && q.y == y) v = add_r(v, ldc(amp, i));
   }
   return v;
 }
@@ -509,14 +537,17 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
           }
 #pragma unroll
           for (int v = 0; v < VX; v++) {
-            float a = t.c0 * q[f][j][v][kH + p];
-#pragma unroll
-            for (int k = 1; k <= kH; k++) a += t.cx[k - 1] * (xs[kH + v - k] + xs[kH + v + k]);
-#pragma unroll
-            for (int k = 1; k <= kH; k++) a += t.cy[k - 1] * (up[k - 1][v] + dn[k - 1][v]);
+            // centre, then the x, y and z pairs, each pair summed first
+            float a = mul_r(t.c0, q[f][j][v][kH + p]);
 #pragma unroll
             for (int k = 1; k <= kH; k++)
-              a += t.cz[k - 1] * (q[f][j][v][kH + p - k] + q[f][j][v][kH + p + k]);
+              a = fma_r(t.cx[k - 1], add_r(xs[kH + v - k], xs[kH + v + k]), a);
+#pragma unroll
+            for (int k = 1; k <= kH; k++)
+              a = fma_r(t.cy[k - 1], add_r(up[k - 1][v], dn[k - 1][v]), a);
+#pragma unroll
+            for (int k = 1; k <= kH; k++)
+              a = fma_r(t.cz[k - 1], add_r(q[f][j][v][kH + p - k], q[f][j][v][kH + p + k]), a);
             lap[v][f] = a;
             ctr[v][f] = q[f][j][v][kH + p];
           }

@@ -22,6 +22,7 @@
 
 #include "born_device.h"
 #include "born_launch.h"
+#include "born_pair.h"
 
 namespace born {
 
@@ -68,7 +69,11 @@ struct FwdOp {
   static constexpr bool DAMP = true;
   const T *__restrict__ s1;
   const T *__restrict__ r1;
-  T *__restrict__ s0;
+  // old s level (read) and new s level (written): the same buffer (in
+  // place), or with scatter 0 and a kept history two levels of the history,
+  // whose level k is s of level k+1
+  const T *s0;
+  T *s_new;
   T *__restrict__ r0;
   const float *__restrict__ vel;
   const float *__restrict__ model;
@@ -112,7 +117,7 @@ struct FwdOp {
       ring[f].scale[sl.nxt] = b.sn[f];
       ring[f].maxb[sl.clr] = 0u;
     }
-    if (hist) *hscale = b.sh;
+    if (hscale) *hscale = b.sh;
   }
   __device__ void publish_max(int f, float m) const {
     atomicMax(&ring[f].maxb[sl.nxt], __float_as_uint(m));
@@ -133,7 +138,7 @@ struct FwdOp {
   }
   template <int VX>
   __device__ void storev(const Blk2 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
-    st_vec<VX>(s0, o, out[0], b.sn[0]);
+    st_vec<VX>(s_new, o, out[0], b.sn[0]);
     st_vec<VX>(r0, o, out[1], b.sn[1]);
     if (hist) st_vec<VX>(hist, o, out[2], b.sh);
   }
@@ -141,19 +146,11 @@ struct FwdOp {
                        const float *lap, const float *ctr, const float *p,
                        float (&out)[NOUT], float (&tmax)[NF]) const {
     const float v = p[2], bb = c.b;
-    float sn = bb * (v * lap[0] + ctr[0] + ctr[0] - bb * p[0]);
+    float sn = leap(bb, v, lap[0], ctr[0], p[0]);
     if (pl.pm) sn = add_points(pts, pl.pm, c.x, c.y, sn, amp);
-    float rn = bb * (v * lap[1] + ctr[1] + ctr[1] - bb * p[1]);
+    float rn = leap(bb, v, lap[1], ctr[1], p[1]);
     float sig;
-    if (SCATTER == 0) {
-      // the reference's double literal: evaluated in double, rounded once
-      // (cpu_prop.cc:220)
-      rn = (float)((double)rn + .000001 * (double)sn * (double)p[3]);
-      sig = sn;
-    } else {
-      sig = bb * lap[0];
-      rn = rn + p[3] * sig;
-    }
+    born_scatter<SCATTER>(bb, lap[0], sn, p[3], rn, sig);
     out[0] = sn;
     out[1] = rn;
     out[2] = sig;
@@ -171,7 +168,7 @@ struct FwdOp {
         for (int k = 0; k < 8; k++) v[k] = q[(long long)k * rm.nrec];
 #pragma unroll
         for (int k = 0; k < 8; k++)
-          q[(long long)k * rm.nrec] = __fadd_rn(v[k], __fmul_rn(rn, ldc(tab, k)));
+          q[(long long)k * rm.nrec] = add_r(v[k], mul_r(rn, ldc(tab, k)));
       }
     }
   }
@@ -273,11 +270,10 @@ struct RadjOp {
       sn = p[0];
     }
     const float b = c.b;
-    float rn = r_prop ? b * (p[3] * lap[1] + ctr[1] + ctr[1] - b * p[1]) : b * (b * p[1]);
+    float rn = r_prop ? leap(b, p[3], lap[1], ctr[1], p[1]) : mul_r(b, mul_r(b, p[1]));
     if (pl.slot >= 0 && inject) {
       for (int i = rec_first(rm, pl.slot, c.x, c.y, nx, ny); i >= 0; i = rm.rnext[i])
-        rn = __fadd_rn(rn, __fmul_rn(sc, sinc8(tab, tr + (long long)id * rm.nrec + i,
-                                               rm.nrec)));
+        rn = add_r(rn, mul_r(sc, sinc8(tab, tr + (long long)id * rm.nrec + i, rm.nrec)));
     }
     out[0] = sn;
     out[1] = rn;
@@ -345,11 +341,11 @@ struct SrecOp {
                        const float *ctr, const float *p, float (&out)[NOUT],
                        float (&tmax)[NF]) const {
     const float bb = c.b;
-    float sn = bb * (p[1] * lap[0] + ctr[0] + ctr[0] - bb * p[0]);
+    float sn = leap(bb, p[1], lap[0], ctr[0], p[0]);
     if (pm) sn = add_points(pts, pm, c.x, c.y, sn, amp);
     if constexpr (H) tmax[0] = fmaxf(tmax[0], fabsf(sn));
     out[0] = sn;
-    out[1] = SCATTER == 0 ? sn : bb * lap[0];
+    out[1] = SCATTER == 0 ? sn : mul_r(bb, lap[0]);
   }
 };
 
@@ -419,7 +415,7 @@ struct LamOp {
                        const float *ctr, const float *p, float (&out)[NOUT],
                        float (&tmax)[NF]) const {
     const float bb = c.b, v = p[1];
-    float un = bb * (v * lap[0] + ctr[0] + ctr[0] - bb * p[0]);
+    float un = leap(bb, v, lap[0], ctr[0], p[0]);
     if (slot >= 0 && inject) {
       float g = 0.f;
       for (int i = rec_first(rm, slot, c.x, c.y, nx, ny); i >= 0; i = rm.rnext[i])
@@ -460,6 +456,7 @@ struct BatchFwd {
     o.s1 = base.s1 + d.cell0;
     o.r1 = base.r1 + d.cell0;
     o.s0 = base.s0 + d.cell0;
+    o.s_new = base.s_new + d.cell0;
     o.r0 = base.r0 + d.cell0;
     o.vel = base.vel + d.cell0;
     o.model = base.model + d.cell0;
@@ -538,7 +535,7 @@ __global__ void src_amp_kernel(float *amp, int nsteps, int npts, const float *sr
   const int ii = tt - id * jt;
   float a = 0.f;
   if (id + 7 < ntsrc && ii < ntab)
-    a = __fmul_rn(scale, sinc8(tab + (long long)ii * nsinc, src + (long long)i * ntsrc + id, 1));
+    a = mul_r(scale, sinc8(tab + (long long)ii * nsinc, src + (long long)i * ntsrc + id, 1));
   amp[(long long)s * stride + i] = a;
 }
 
@@ -625,6 +622,7 @@ hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a
   auto fill = [&](auto &op, auto *tp) {
     using T = std::remove_pointer_t<decltype(tp)>;
     op.s1 = as<T>(a.s1); op.r1 = as<T>(a.r1); op.s0 = as<T>(a.s0); op.r0 = as<T>(a.r0);
+    op.s_new = as<T>(a.s_new ? a.s_new : a.s0);
     op.vel = a.vel; op.model = a.model; op.hist = as<T>(a.hist);
     op.pts = to_points(a.src); op.amp = a.amp;
     op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
@@ -647,6 +645,51 @@ hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a
   }
   FwdOp<1, float> op{};
   return fill(op, (float *)nullptr);
+}
+
+const PairShape kPairShapes[kNumPairShapes] = {{2, 8}, {1, 8}, {2, 4}, {4, 4}};
+
+template <int SCATTER, int RY, int WY>
+static hipError_t launch_pair_shape(const Geometry &geo, int zc, const PairOp<SCATTER> &op,
+                                    hipStream_t st) {
+  const Grid g = pair_grid(geo.nx, geo.ny, geo.nz, RY * WY, zc > 0 ? zc : 64);
+  const int nb = g.ntx * g.nty * g.ntz;
+  hipLaunchKernelGGL((pair_kernel<SCATTER, RY, WY>), dim3(nb), dim3(kTX * WY), 0, st, g,
+                     geo.taps, geo.sponge, op);
+  return hipGetLastError();
+}
+
+template <int SCATTER>
+static hipError_t launch_pair(const Geometry &geo, const Tune &tu, const PairOp<SCATTER> &op,
+                              hipStream_t st) {
+  if (geo.nx < 2 * kH + 1 || geo.ny < 2 * kH + 1 || geo.nz < 2 * kH + 1)
+    return hipErrorInvalidValue;
+  switch (tu.shape) {
+    case 0: return launch_pair_shape<SCATTER, 2, 8>(geo, tu.zchunk, op, st);
+    case 1: return launch_pair_shape<SCATTER, 1, 8>(geo, tu.zchunk, op, st);
+    case 2: return launch_pair_shape<SCATTER, 2, 4>(geo, tu.zchunk, op, st);
+    case 3: return launch_pair_shape<SCATTER, 4, 4>(geo, tu.zchunk, op, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_pair_step(const Geometry &geo, const Tune &tu, const PairStep &a,
+                            hipStream_t st) {
+  auto fill = [&](auto &op) {
+    op.s_old = a.s_old; op.s_cur = a.s_cur; op.r_old = a.r_old; op.r_cur = a.r_cur;
+    op.s_a = a.s_a; op.r_a = a.r_a; op.s_b = a.s_b; op.r_b = a.r_b;
+    op.h_a = a.h_a; op.h_b = a.h_b; op.vel = a.vel; op.model = a.model;
+    op.pts = to_points(a.src); op.amp_a = a.amp_a; op.amp_b = a.amp_b;
+    op.rm = a.rec; op.tr = a.tr; op.tab_a = a.tab_a; op.tab_b = a.tab_b;
+    op.id_a = a.id_a; op.id_b = a.id_b; op.ext_a = a.ext_a; op.ext_b = a.ext_b;
+    return launch_pair(geo, tu, op, st);
+  };
+  if (a.scatter == 0) {
+    PairOp<0> op{};
+    return fill(op);
+  }
+  PairOp<1> op{};
+  return fill(op);
 }
 
 hipError_t launch_src_step(const Geometry &geo, const Tune &tu, const SrcStep &a,
@@ -791,6 +834,7 @@ hipError_t launch_batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFw
     using T = std::remove_pointer_t<decltype(tp)>;
     auto &op = bop.base;
     op.s1 = as<T>(a.s1); op.r1 = as<T>(a.r1); op.s0 = as<T>(a.s0); op.r0 = as<T>(a.r0);
+    op.s_new = as<T>(a.s0);
     op.vel = a.vel; op.model = a.model; op.hist = nullptr;
     op.pts = to_points(a.src); op.amp = a.amp;
     op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;

@@ -47,6 +47,7 @@ struct Half {
 struct FwdStep {
   const float *s1, *r1;
   float *s0, *r0;
+  float *s_new;  // where the new s level goes (null: s0, in place)
   const float *vel, *model;
   float *hist;
   SrcPts src;
@@ -60,6 +61,29 @@ struct FwdStep {
   const unsigned *mmax;
   float *hscale;
 };
+
+// Two forward steps in one launch (FP32 levels): reads levels k-1 and k of
+// s and r, writes levels k+1 (s_a, r_a) and k+2 (s_b, r_b); four distinct
+// buffers per field.  *_a belong to step it, *_b to step it+1.
+struct PairStep {
+  const float *s_old, *s_cur, *r_old, *r_cur;
+  float *s_a, *r_a, *s_b, *r_b;
+  float *h_a, *h_b;  // optional history levels it and it+1
+  const float *vel, *model;
+  SrcPts src;
+  const float *amp_a, *amp_b;
+  RecMap rec;
+  float *tr;
+  const float *tab_a, *tab_b;
+  int id_a, id_b, ext_a, ext_b;
+  int scatter;
+};
+// Pair kernel tile: RY tile rows per thread, WY wavefronts (56 x RY*WY cells).
+struct PairShape {
+  int ry, wy;
+};
+constexpr int kNumPairShapes = 4;
+extern const PairShape kPairShapes[kNumPairShapes];
 
 struct SrcStep {
   const float *s1;
@@ -146,6 +170,8 @@ struct BatchFwdStep {
 
 hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a,
                            hipStream_t st);
+hipError_t launch_pair_step(const Geometry &geo, const Tune &tu, const PairStep &a,
+                            hipStream_t st);
 hipError_t launch_batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
                                  hipStream_t st);
 hipError_t launch_batch_ring_init(float *rings, int nshots, hipStream_t st);

@@ -56,6 +56,8 @@ SIGNATURES = {
     "born_last_checkpoint_plan": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "born_set_zchunk": (_i, [_vp, _i]),
     "born_set_storage": (_i, [_vp, _i]),
+    "born_set_step_pairs": (_i, [_vp, _i]),
+    "born_last_step_pairs": (_i, [_vp]),
     "born_set_autotune": (_i, [_vp, _i]),
     "born_get_tuning": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
                              C.POINTER(C.c_float)]),
@@ -231,13 +233,23 @@ class Propagator:
 
     KERNELS = ("fwd", "fwd_dV", "source", "rtm_adjoint", "background", "background_dV",
                "adjoint_state", "fwd_f16", "fwd_dV_f16", "background_f16",
-               "background_dV_f16", "adjoint_state_f16")
+               "background_dV_f16", "adjoint_state_f16", "fwd_pair", "fwd_dV_pair")
     STORAGE = {"f32": 0, "f16": 1}
 
     def set_storage(self, kind):
         """Wavefield storage of the Born forward / exact adjoint: "f32" or
         "f16" (FP16 levels with per-level power-of-two scales)."""
         self._chk(self.lib.born_set_storage(self.h, self.STORAGE[kind]), "storage")
+
+    def set_step_pairs(self, mode):
+        """Two time steps per launch in the FP32 Born forward: 0 / False
+        (default) single steps, 1 pairs unless timed slower, 2 / True pairs."""
+        mode = {True: 2, False: 0}.get(mode, mode) if isinstance(mode, bool) else int(mode)
+        self._chk(self.lib.born_set_step_pairs(self.h, mode), "step_pairs")
+
+    @property
+    def last_step_pairs(self):
+        return bool(self.lib.born_last_step_pairs(self.h))
 
     def tuning(self):
         out = {}
