@@ -6,8 +6,10 @@ Workload (the 256^3 Born fwd+adj pair at 10 m): 304^3 padded domain, 3 shots,
 shot, the Born forward (data from the model) and the exact adjoint (image
 from those data), all inputs resident in HBM.  Work unit: a Born cell-step =
 one interior cell advanced one time step through the forward (2 wavefield
-updates) and the adjoint (2 wavefield updates); Mcell-updates/s = 4 x Born
-cell-steps/s.
+updates) and the adjoint (2 wavefield updates: background and adjoint
+state); Mcell-updates/s = 4 x Born cell-steps/s.  The unit is fixed per Born
+cell-step: with the background history kept in HBM the adjoint advances only
+the adjoint state, and the work counted is still 4 updates per cell-step.
 
 Prints one JSON line (see the keys below).  --gpus N with N > 1 is refused:
 the path shards by shot and the single-GPU number is the one measured.
@@ -26,15 +28,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "sc17-born_amd")
 
 HBM_PEAK_GBS = 8000.0
-FWD_BYTES_PER_CELL = 32      # s and r: read cur, read old, write new; vel; model
-ADJ_BYTES_PER_CELL = 36      # background recompute 12, adjoint state 12, vel 4, image r/w 8
-F16_BYTES_PER_CELL = 44      # the same with 2-byte wavefield levels (vel, model, image FP32)
+# Algorithmic HBM bytes per interior cell and step, FP32 / FP16 levels
+# (vel, model, image and traces stay FP32):
+#   forward: s, r read cur + old, vel, model; s, r written (s into the kept
+#     history with scatter 0)                                   32 / 20
+#   adjoint state: nu read cur + old, vel, sigma, image; nu and image
+#     written                                                   28 / 20
+#   background recompute (steps of checkpoint segments): s read cur + old,
+#     vel; s and sigma written                                  20 / 12
+FWD_BYTES_PER_CELL = 32
+BYTES = {"f32": {"fwd": 32, "lam": 28, "srec": 20}, "f16": {"fwd": 20, "lam": 20, "srec": 12}}
+
+
+def born_bytes(storage, nt, recomputed):
+    """Algorithmic bytes per Born cell-step of one forward + exact adjoint,
+    for the checkpoint plan that ran (recomputed background steps)."""
+    b = BYTES[storage]
+    return b["fwd"] + b["lam"] + b["srec"] * recomputed / nt
 METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs FP32 plain"
 # HBM bytes per launch of the forward step kernel on the 304^3 workload, from
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of this script
 # (profiles/born_fwd_adj_304cube_hbm_counters.md); reported as roofline.traffic
 # only when the measured workload and kernel are the profiled ones.
-PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 2, 8, 1, 1, 64", "f32"): 1098.4e6}
+PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 1, 8, 1, 1, 64", "f32"): 914.2e6,
+                    ("c2", "FwdOp<0, float>, 2, 8, 1, 1, 64", "f32"): 939.8e6}
 
 
 def load(name):
@@ -199,6 +216,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-shots leg")
     ap.add_argument("--batch-nt", type=int, default=0, help="time steps of the batch leg")
     ap.add_argument("--batch-shots", type=int, default=0, help="shots of the batch leg")
+    ap.add_argument("--fwd-shape", default="",
+                    help="fix the FP32 forward launch shape: rows,waves,cells_per_lane "
+                         "(64-cell rows; profiling runs), e.g. 1,8,1")
+    ap.add_argument("--step-pairs", type=int, default=0,
+                    help="FP32 forward in two-step launches: 0 off, 1 when timed faster, 2 on")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (a kernel of known traffic "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -241,6 +263,10 @@ def main():
                              np.zeros((p.nrec, p.nt_big), np.float32))
     if args.zchunk:
         P.set_zchunk(args.zchunk)
+    P.set_step_pairs(args.step_pairs)
+    if args.fwd_shape:
+        ry, wy, vx = (int(v) for v in args.fwd_shape.split(","))
+        P.set_tuning("fwd", ry, wy, vx, 1, args.zchunk or 64)
     if args.budget_gb:
         P.set_checkpoint_budget(int(args.budget_gb * 2 ** 30))
     model = torch.from_numpy(p.model).to(dev)
@@ -305,7 +331,7 @@ def main():
     fwd_launch_s = fwd_ms / 1e3 / nt
     fwd_alg_bytes = FWD_BYTES_PER_CELL * p.interior_cells
     achieved = fwd_alg_bytes / fwd_launch_s / 1e9
-    alg_gbs = (FWD_BYTES_PER_CELL + ADJ_BYTES_PER_CELL) * cell_steps / elapsed / 1e9
+    alg_gbs = born_bytes("f32", nt, recomputed) * cell_steps / elapsed / 1e9
     ft = tuning["fwd"]
     fwd_kernel = "FwdOp<0, float>, %d, %d, %d, %d, %d" % (
         ft["rows_per_thread"], ft["waves"], ft["cells_per_lane"], ft["planes_per_step"],
@@ -333,6 +359,7 @@ def main():
             "global_batch": nshots,
             "parallelism": "single GPU",
             "born_cell_steps_per_s": round(cell_steps / elapsed, 1),
+            "bytes_per_born_cell_step": round(born_bytes("f32", nt, recomputed), 2),
             "algorithmic_GBps": round(alg_gbs, 1),
             "algorithmic_frac_of_hbm": round(alg_gbs / HBM_PEAK_GBS, 4),
             "forward_ms_per_shot": round(float(fwd_ms), 2),
@@ -365,7 +392,8 @@ def main():
             "adjoint_ms_per_shot": round(float(np.mean([a.elapsed_time(b) for a, b in ae16])), 2),
             "checkpoint_segments": segs16,
             "recomputed_steps": rec16,
-            "bytes_per_born_cell_step": F16_BYTES_PER_CELL,
+            "bytes_per_born_cell_step": round(born_bytes("f16", nt, rec16), 2),
+            "algorithmic_GBps": round(born_bytes("f16", nt, rec16) * cell_steps / el16 / 1e9, 1),
         }
     P.close()
     del model, traces, img
