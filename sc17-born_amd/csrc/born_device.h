@@ -291,7 +291,11 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   constexpr int LW = TW + 2 * kH;
   constexpr int LH = TY + 2 * kH;
   constexpr int NQ = 2 * kH + PZ;  // queue: planes z - kH .. z + kH + PZ - 1
-  __shared__ __align__(16) float tile[PZ][NF][LH][LW];
+  // LDS planes form a ring of NR = kH + PZ slots (plane q in slot q % NR):
+  // a plane's halo is written when the plane enters the z queue as a lead,
+  // kH planes before its centre rows and its use
+  constexpr int NR = kH + PZ;
+  __shared__ __align__(16) float tile[NR][NF][LH][LW];
 
   const int bx = L % g.ntx;
   const int rest = L / g.ntx;
@@ -370,10 +374,18 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   // The LDS halo of a plane is loaded together with the plane's centre rows
   // (when the plane enters the queue as a lead), not kH planes later: the
   // neighbouring tiles read those rows at about the same time, so they come
-  // from L2.  hq[k] holds the halo of plane z + k.
+  // from L2.  It waits in its LDS ring slot, not in registers; hl holds the
+  // halos of the leads in flight.
   float q[NF][RY][VX][NQ];
   float lead[PZ][NF][RY][VX];
-  float hq[kH + PZ][NF][HPT][VX];
+  float hl[PZ][NF][HPT][VX];
+  auto put_halo = [&](int slot, int f, int k, const float (&h)[VX]) {
+    float *d = &tile[slot][f][hr[k]][hc[k]];
+    if constexpr (VX == 1)
+      d[0] = h[0];
+    else
+      *(float2 *)d = make_float2(h[0], h[1]);
+  };
   float pre[PZ][RY][NP][VX], pre_n[PZ][RY][NP][VX];
   float tmax[NF];
 #pragma unroll
@@ -398,11 +410,14 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
       const int zp = min(z0 + p, g.nz - 1);
 #pragma unroll
       for (int k = 0; k < HPT; k++) {
-        if (hv[k]) {
-          op.template ldv<VX>(blk, f, (uint32_t)zp * g.plane + hoff[k], hq[p][f][k]);
+        if (p < kH) {
+          if (hv[k]) {
+            float h[VX];
+            op.template ldv<VX>(blk, f, (uint32_t)zp * g.plane + hoff[k], h);
+            put_halo((z0 + p) % NR, f, k, h);
+          }
         } else {
-#pragma unroll
-          for (int v = 0; v < VX; v++) hq[p][f][k][v] = 0.f;
+          op.template ldv<VX>(blk, f, (uint32_t)zp * g.plane + hoff[k], hl[p - kH][f][k]);
         }
       }
     }
@@ -444,7 +459,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
           for (int v = 0; v < VX; v++) q[f][j][v][2 * kH + p] = lead[p][f][j][v];
 #pragma unroll
         for (int p = 0; p < PZ; p++) {
-          float *d = &tile[p][f][kH + ty + NG * j][kH + VX * tx];
+          float *d = &tile[(z + p) % NR][f][kH + ty + NG * j][kH + VX * tx];
           if constexpr (VX == 1)
             d[0] = q[f][j][0][kH + p];
           else
@@ -454,24 +469,9 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
 #pragma unroll
       for (int p = 0; p < PZ; p++)
 #pragma unroll
-        for (int k = 0; k < HPT; k++) {
-          if (hv[k]) {
-            float *d = &tile[p][f][hr[k]][hc[k]];
-            if constexpr (VX == 1)
-              d[0] = hq[p][f][k][0];
-            else
-              *(float2 *)d = make_float2(hq[p][f][k][0], hq[p][f][k][1]);
-          }
-        }
-    }
-#pragma unroll
-    for (int p = 0; p < kH; p++)
-#pragma unroll
-      for (int f = 0; f < NF; f++)
-#pragma unroll
         for (int k = 0; k < HPT; k++)
-#pragma unroll
-          for (int v = 0; v < VX; v++) hq[p][f][k][v] = hq[p + PZ][f][k][v];
+          if (hv[k]) put_halo((z + kH + p) % NR, f, k, hl[p][f][k]);
+    }
     flush();
     // the next step's loads, in flight during this step's compute: queue
     // leads z+PZ+kH.. with their halos, and the pointwise inputs of z+PZ..
@@ -485,8 +485,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
 #pragma unroll
           for (int j = 0; j < RY; j++) op.template ldv<VX>(blk, f, zl + row[j], lead[p][f][j]);
 #pragma unroll
-          for (int k = 0; k < HPT; k++)
-            if (hv[k]) op.template ldv<VX>(blk, f, zl + hoff[k], hq[kH + p][f][k]);
+          for (int k = 0; k < HPT; k++) op.template ldv<VX>(blk, f, zl + hoff[k], hl[p][f][k]);
         }
 #pragma unroll
         for (int j = 0; j < RY; j++) op.template prefetchv<VX>(blk, pre_n[p][j], zn + row[j]);
@@ -497,6 +496,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     for (int p = 0; p < PZ; p++) {
       const int zz = z + p;
       if (PZ > 1 && zz >= z1) break;
+      const int sl = zz % NR;
       const uint32_t zo = (uint32_t)zz * g.plane;
       const auto pl = op.plane(blk, zz);
       const float bzv = Op::DAMP ? ldc(sp.bz, zz) : 1.f;
@@ -512,23 +512,23 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
           float up[kH][VX], dn[kH][VX];
           if constexpr (VX == 1) {
 #pragma unroll
-            for (int i = 0; i < 2 * kH + 1; i++) xs[i] = tile[p][f][lr][lc - kH + i];
+            for (int i = 0; i < 2 * kH + 1; i++) xs[i] = tile[sl][f][lr][lc - kH + i];
 #pragma unroll
             for (int k = 1; k <= kH; k++) {
-              up[k - 1][0] = tile[p][f][lr - k][lc];
-              dn[k - 1][0] = tile[p][f][lr + k][lc];
+              up[k - 1][0] = tile[sl][f][lr - k][lc];
+              dn[k - 1][0] = tile[sl][f][lr + k][lc];
             }
           } else {
 #pragma unroll
             for (int i = 0; i < kH + 1; i++) {
-              const float2 w = *(const float2 *)&tile[p][f][lr][lc - kH + 2 * i];
+              const float2 w = *(const float2 *)&tile[sl][f][lr][lc - kH + 2 * i];
               xs[2 * i] = w.x;
               xs[2 * i + 1] = w.y;
             }
 #pragma unroll
             for (int k = 1; k <= kH; k++) {
-              const float2 a = *(const float2 *)&tile[p][f][lr - k][lc];
-              const float2 b = *(const float2 *)&tile[p][f][lr + k][lc];
+              const float2 a = *(const float2 *)&tile[sl][f][lr - k][lc];
+              const float2 b = *(const float2 *)&tile[sl][f][lr + k][lc];
               up[k - 1][0] = a.x;
               up[k - 1][1] = a.y;
               dn[k - 1][0] = b.x;
