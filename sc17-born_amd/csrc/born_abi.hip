@@ -453,11 +453,15 @@ float *nth(float *base, long long k, long long n, size_t esz) {
   return (float *)((char *)base + (size_t)k * n * esz);
 }
 
-// ctx->d_ring (64 words): rings s, r, u at 0, 8, 16 (scale[4], maxb[4]);
-// max |model| at 24, max |trace| at 25; scratch rings for the autotuner at
-// 32, 40, with scratch max at 48 and scratch history scale at 49.
-enum { RING_S = 0, RING_R = 8, RING_U = 16, MAX_M = 24, MAX_TR = 25, RING_T0 = 32,
-       RING_T1 = 40, MAX_T = 48, HS_T = 49 };
+// ctx->d_ring (64 words): rings s, r at 0, 8 (scale[4], maxb[4]) and max
+// |model| at 16 (the forward's ring block); ring u at 24, max |trace| at 32;
+// the autotuner's scratch ring block at 40 (rings at 40, 48, max at 56) and
+// scratch history scale at 57.
+enum { RING_S = 0, RING_R = 8, MAX_M = 16, RING_U = 24, MAX_TR = 32, RING_T0 = 40,
+       RING_T1 = 48, MAX_T = 56, HS_T = 57 };
+static_assert(RING_R == RING_S + 8 && MAX_M == RING_S + 16 && RING_T1 == RING_T0 + 8 &&
+                  MAX_T == RING_T0 + 16 && HS_T < 64,
+              "ring blocks");
 born::Ring ring(born_ctx *c, int at) {
   return {c->d_ring + at, (unsigned *)(c->d_ring + at + 4)};
 }
@@ -843,8 +847,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     t.vel = ctx->d_vel2; t.model = model; t.src = no_points(); t.rec = recmap(ctx);
     t.tr = traces; t.tab = ctx->d_tab_d; t.extract = 0; t.scatter = scatter;
     t.h = half_state(ctx, 0);
-    t.ring[0] = ring(ctx, RING_T0); t.ring[1] = ring(ctx, RING_T1);
-    t.mmax = maxp(ctx, MAX_T); t.hscale = ctx->d_ring + HS_T;
+    t.rg = ctx->d_ring + RING_T0; t.hscale = ctx->d_ring + HS_T;
     CHK(autotune(ctx, kind, [&](const born::Tune &tu) {
       return born::launch_fwd_step(geo, tu, t, ctx->stream); }));
   }
@@ -916,9 +919,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   a.rec = recmap(ctx);
   a.tr = traces;
   a.scatter = scatter;
-  a.ring[0] = ring(ctx, RING_S);
-  a.ring[1] = ring(ctx, RING_R);
-  a.mmax = maxp(ctx, MAX_M);
+  a.rg = ctx->d_ring + RING_S;
   // With scatter 0 the scatter operand of step it is s of level it+1, so a
   // kept history holds the s levels: step it reads levels it (history it-1)
   // and it-1 (history it-2) and writes level it+1 into history it, one

@@ -84,11 +84,18 @@ struct FwdOp {
   float *__restrict__ tr;  // traces, time-major [ntb][nrec]
   const float *__restrict__ tab;
   int id, extract, nx, ny;
-  Ring ring[2];  // FP16: s, r
-  Slots sl;
+  // FP16 scale state in as few scalar registers as possible (the loop needs
+  // none of it, but kernel arguments stay live in scalar registers, and
+  // too many of them spill into vector lanes that every use reloads): the
+  // ring block rg holds the s ring at 0, the r ring at 8 and max |model| at
+  // 16 (float bits); slots packs cur, old, nxt, clr in its four bytes.
+  float *rg;
+  int slots;
   float gain, tapsum;
-  const unsigned *mmax;  // max |model| as float bits
-  float *hscale;         // FP16: scale of this step's history level
+  float *hscale;  // FP16: scale of this step's history level
+
+  __device__ Ring ring(int f) const { return {rg + 8 * f, (unsigned *)(rg + 8 * f + 4)}; }
+  __device__ int slot(int i) const { return (slots >> (8 * i)) & 0xff; }
 
   __device__ Blk2 block(int x0, int x1, int y0, int y1, int z0, int z1) const {
     Blk2 b;
@@ -96,15 +103,16 @@ struct FwdOp {
     for (int f = 0; f < 2; f++) b.ic[f] = b.io[f] = b.sn[f] = 1.f;
     b.sh = 1.f;
     if constexpr (H) {
+      const int cur = slot(0), old = slot(1);
       for (int f = 0; f < 2; f++) {
-        b.ic[f] = 1.f / ring[f].scale[sl.cur];
-        b.io[f] = 1.f / ring[f].scale[sl.old];
+        b.ic[f] = 1.f / ring(f).scale[cur];
+        b.io[f] = 1.f / ring(f).scale[old];
       }
-      const float ms = ring_max(ring[0], sl.cur);
-      const float bs = gain * ms + ring_max(ring[0], sl.old) + amp_sum(pts, amp);
-      const float mm = __uint_as_float(*mmax);
+      const float ms = ring_max(ring(0), cur);
+      const float bs = gain * ms + ring_max(ring(0), old) + amp_sum(pts, amp);
+      const float mm = __uint_as_float(*(const unsigned *)(rg + 16));
       const float sig = SCATTER == 0 ? bs : tapsum * ms;
-      const float br = gain * ring_max(ring[1], sl.cur) + ring_max(ring[1], sl.old) +
+      const float br = gain * ring_max(ring(1), cur) + ring_max(ring(1), old) +
                        (SCATTER == 0 ? 1e-6f * bs * mm : mm * sig);
       b.sn[0] = level_scale(bs);
       b.sn[1] = level_scale(br);
@@ -114,13 +122,13 @@ struct FwdOp {
   }
   __device__ void first(const Blk2 &b) const {
     for (int f = 0; f < 2; f++) {
-      ring[f].scale[sl.nxt] = b.sn[f];
-      ring[f].maxb[sl.clr] = 0u;
+      ring(f).scale[slot(2)] = b.sn[f];
+      ring(f).maxb[slot(3)] = 0u;
     }
     if (hscale) *hscale = b.sh;
   }
   __device__ void publish_max(int f, float m) const {
-    atomicMax(&ring[f].maxb[sl.nxt], __float_as_uint(m));
+    atomicMax(&ring(f).maxb[slot(2)], __float_as_uint(m));
   }
   template <int VX>
   __device__ void ldv(const Blk2 &b, int f, uint32_t o, float (&v)[VX]) const {
@@ -471,10 +479,7 @@ struct BatchFwd {
     o.tr = base.tr + d.tr0;
     o.nx = d.nx;
     o.ny = d.ny;
-    float *rg = rings + d.ring0;
-    o.ring[0] = {rg, (unsigned *)(rg + 4)};
-    o.ring[1] = {rg + 8, (unsigned *)(rg + 12)};
-    o.mmax = (const unsigned *)(rg + 16);
+    o.rg = rings + d.ring0;
     o.gain = d.gain;
     return r;
   }
@@ -611,6 +616,7 @@ static hipError_t launch(const Geometry &geo, const Tune &tu, const Op &op, hipS
 }
 
 static Points to_points(const SrcPts &s) { return {(const int4 *)s.p, s.n}; }
+static int pack_slots(const Slots &s) { return s.cur | s.old << 8 | s.nxt << 16 | s.clr << 24; }
 
 template <class T>
 static T *as(const void *p) {
@@ -627,8 +633,8 @@ hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a
     op.pts = to_points(a.src); op.amp = a.amp;
     op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
     op.extract = a.extract; op.nx = geo.nx; op.ny = geo.ny;
-    op.ring[0] = a.ring[0]; op.ring[1] = a.ring[1]; op.sl = a.h.sl;
-    op.gain = a.h.gain; op.tapsum = a.h.tapsum; op.mmax = a.mmax; op.hscale = a.hscale;
+    op.rg = a.rg; op.slots = pack_slots(a.h.sl);
+    op.gain = a.h.gain; op.tapsum = a.h.tapsum; op.hscale = a.hscale;
     return launch(geo, tu, op, st);
   };
   if (a.h.on) {
@@ -839,7 +845,7 @@ hipError_t launch_batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFw
     op.pts = to_points(a.src); op.amp = a.amp;
     op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
     op.extract = a.extract;
-    op.sl = a.h.sl; op.tapsum = a.h.tapsum; op.hscale = nullptr;
+    op.slots = pack_slots(a.h.sl); op.tapsum = a.h.tapsum; op.hscale = nullptr;
     bop.sponge = b.sponge; bop.rings = b.rings; bop.desc = b.desc; bop.step = b.step;
     bop.npts_total = b.npts_total;
     return launch_batch(taps, tu, b, bop, st);

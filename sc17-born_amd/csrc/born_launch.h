@@ -57,8 +57,7 @@ struct FwdStep {
   const float *tab;
   int id, extract, scatter;
   Half h;
-  Ring ring[2];
-  const unsigned *mmax;
+  float *rg;  // FP16 ring block: s ring at 0, r ring at 8, max |model| at 16
   float *hscale;
 };
 
