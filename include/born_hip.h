@@ -158,14 +158,15 @@ int born_last_checkpoint_plan(const born_ctx *ctx, int *segments,
 #define BORN_STORAGE_F16 1
 int born_set_storage(born_ctx *ctx, int storage);
 
-/* Two time steps per launch in the Born forward with FP32 storage: one
- * sweep reads levels k-1 and k once and writes levels k+1 and k+2; the
- * values are those of single steps.  mode 0 (default): single steps;
- * 1: pairs unless the autotuner timed two single steps faster; 2: pairs.
- * No reference counterpart: the reference advances one step per pass
- * (base/cpu_prop.cc:31-53). */
+/* Two time steps per launch with FP32 storage, in the Born forward and in
+ * the adjoint-state sweep of the exact adjoint: one sweep reads levels k-1
+ * and k once and writes levels k+1 and k+2; the values are those of single
+ * steps.  mode 0 (default): single steps; 1: pairs unless the autotuner
+ * timed two single steps faster; 2: pairs.  No reference counterpart: the
+ * reference advances one step per pass (base/cpu_prop.cc:31-53, 71-95). */
 int born_set_step_pairs(born_ctx *ctx, int mode);
-/* 1 when the last Born forward advanced in two-step launches */
+/* bit 0: the last Born forward advanced in two-step launches; bit 1: the
+ * last exact adjoint did */
 int born_last_step_pairs(const born_ctx *ctx);
 
 /* z planes per block of the step kernels (0 = chosen by the autotuner) */
@@ -179,7 +180,7 @@ int born_set_autotune(born_ctx *ctx, int on);
 /* The chosen launch shape of a step kernel: 0 Born forward, 1 Born forward
  * (dV scatter), 2 source propagation, 3 reference adjoint, 4/5 background
  * recompute, 6 adjoint state; 7-11 the FP16-storage forms of 0, 1, 4, 5, 6;
- * 12, 13 the two-step forms of 0, 1 (tile 56 x rows*waves).
+ * 12, 13, 14 the two-step forms of 0, 1, 6 (tile 56 x rows*waves).
  * ms = timed duration of one launch (-1 when not tuned). */
 int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
                     int *waves_per_block, int *zchunk, float *ms);

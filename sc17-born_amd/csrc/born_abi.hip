@@ -42,8 +42,8 @@ struct born_ctx {
   float *d_bx = nullptr, *d_by = nullptr, *d_bz = nullptr;
   int zchunk = 0;  // override of the tuned z chunk (0: tuned)
   int autotune = 1;
-  born::Tune tune[14];
-  float tune_ms[14];
+  born::Tune tune[16];
+  float tune_ms[16];
   int pairs = 0;  // two-step forward launches for FP32 levels: 0 off, 1 auto, 2 on
 
   int nsinc_s = 0, jts = 0, nsinc_d = 0, jtd = 0;
@@ -91,6 +91,7 @@ struct born_ctx {
   int hist_scatter = -1, hist_jt = -1, hist_nt = -1, hist_storage = -1;
   int last_segments = 0, last_recomputed = 0;
   int last_pairs = 0;
+  int last_lam_pairs = 0;
 };
 
 namespace {
@@ -180,10 +181,10 @@ born::RecMap recmap(const born_ctx *c) {
 // step kernels that are tuned separately
 enum Kind {
   K_FWD0, K_FWD1, K_SRC, K_RADJ, K_SREC0, K_SREC1, K_LAM,
-  K_FWD0H, K_FWD1H, K_SREC0H, K_SREC1H, K_LAMH, K_FWD0P, K_FWD1P, K_NUM
+  K_FWD0H, K_FWD1H, K_SREC0H, K_SREC1H, K_LAMH, K_FWD0P, K_FWD1P, K_LAMP, K_NUM
 };
-static_assert(K_NUM <= 14, "tuning slots");
-bool is_pair(int kind) { return kind == K_FWD0P || kind == K_FWD1P; }
+static_assert(K_NUM <= 16, "tuning slots");
+bool is_pair(int kind) { return kind == K_FWD0P || kind == K_FWD1P || kind == K_LAMP; }
 
 born::SrcPts points(const born_ctx *c) { return {c->d_pts, c->npts_s}; }
 born::SrcPts no_points() { return {nullptr, 0}; }
@@ -1061,6 +1062,8 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   const long long n = ctx->n;
   if (nt == 0) return BORN_OK;
   const bool half = ctx->storage == BORN_STORAGE_F16;
+  const bool try_pairs = ctx->pairs && !half && nt >= 2;
+  if (try_pairs) CHK(ensure_pair_work(ctx));
   const size_t esz = elem(ctx);
   const Geometry geo = geometry(ctx);
   const int skind = half ? (scatter == 0 ? K_SREC0H : K_SREC1H)
@@ -1088,8 +1091,26 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
     l.trmax = maxp(ctx, MAX_T); l.hscale = ctx->d_ring + HS_T;
     CHK(autotune(ctx, lkind, [&](const born::Tune &tu) {
       return born::launch_lam_step(geo, tu, l, ctx->stream); }));
+    if (try_pairs) {
+      born::LamPairStep lp{};
+      lp.u_old = ctx->d_w[0]; lp.u_cur = ctx->d_w[1]; lp.u_a = ctx->d_w2[0];
+      lp.u_b = ctx->d_w2[1]; lp.vel = ctx->d_vel2; lp.sig_a = lp.sig_b = ctx->d_w[2];
+      lp.acc = ctx->d_acc; lp.rec = recmap(ctx); lp.tr = traces;
+      lp.tab_a = lp.tab_b = ctx->d_tab_d;
+      CHK(autotune_pair(ctx, K_LAMP, [&](const born::Tune &tp) {
+        return born::launch_lam_pair_step(geo, tp, lp, ctx->stream); }));
+    }
   }
   const born::Tune stu = tuning(ctx, skind), ltu = tuning(ctx, lkind);
+  bool lam_pairs = false;
+  if (try_pairs) {
+    // pairs unless both kernels were timed and two single steps won
+    const float mp = ctx->tune_ms[K_LAMP], ms = ctx->tune_ms[lkind];
+    lam_pairs = ctx->pairs == 2 || !(mp > 0.f && ms > 0.f && mp > 2.f * ms);
+    CHK(zero(ctx, ctx->d_w2[0], n));
+    CHK(zero(ctx, ctx->d_w2[1], n));
+  }
+  ctx->last_lam_pairs = lam_pairs;
   const bool cached = ctx->hist_version == ctx->bg_version &&
                       ctx->hist_scatter == scatter && ctx->hist_jt == jt &&
                       ctx->hist_nt == nt && ctx->hist_storage == ctx->storage &&
@@ -1175,6 +1196,14 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   lm.inj = inj;
   lm.trmax = maxp(ctx, MAX_TR);
   int step = 0;  // adjoint steps taken (indexes the ring of nu)
+  // nu levels j-1, j, j+1, j+2 rotate through four buffers in two-step launches
+  float *ub[4] = {u_old, u_cur, ctx->d_w2[0], ctx->d_w2[1]};
+  born::LamPairStep lp{};
+  lp.vel = ctx->d_vel2;
+  lp.acc = ctx->d_acc;
+  lp.rec = recmap(ctx);
+  lp.tr = traces;
+  const born::Tune ptu = tuning(ctx, K_LAMP);
   for (int sg = S - 1; sg >= 0; sg--) {
     const int k0 = sg * L, k1 = std::min(nt, k0 + L);
     if (sg < S - 1) {
@@ -1204,10 +1233,28 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
         recomputed++;
       }
     }
-    for (int k = k1 - 1; k >= k0; k--) {
+    for (int k = k1 - 1; k >= k0;) {
       const int id = k / jt, ii = k - id * jt;
-      lm.u1 = u_cur;
-      lm.u0 = u_old;
+      if (lam_pairs && k - 1 >= k0) {
+        const int idb = (k - 1) / jt, iib = k - 1 - idb * jt;
+        lp.u_old = ub[0]; lp.u_cur = ub[1]; lp.u_a = ub[2]; lp.u_b = ub[3];
+        lp.sig_a = nth(ctx->d_seg, k - k0, n, esz);
+        lp.sig_b = nth(ctx->d_seg, k - 1 - k0, n, esz);
+        lp.tab_a = ctx->d_tab_d + (long long)ii * 8;
+        lp.tab_b = ctx->d_tab_d + (long long)iib * 8;
+        lp.id_a = id;
+        lp.id_b = idb;
+        lp.inj_a = id + 7 < ctx->nt_rec;
+        lp.inj_b = idb + 7 < ctx->nt_rec;
+        HIPCHK(born::launch_lam_pair_step(geo, ptu, lp, ctx->stream));
+        std::swap(ub[0], ub[2]);
+        std::swap(ub[1], ub[3]);
+        step += 2;
+        k -= 2;
+        continue;
+      }
+      lm.u1 = ub[1];
+      lm.u0 = ub[0];
       lm.sig = nth(ctx->d_seg, k - k0, n, esz);
       lm.hscale = half ? ctx->d_hscale + (k - k0) : nullptr;
       lm.tab = ctx->d_tab_d + (long long)ii * 8;
@@ -1215,7 +1262,8 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
       lm.inject = id + 7 < ctx->nt_rec;
       lm.h = half_state(ctx, step++);
       HIPCHK(born::launch_lam_step(geo, ltu, lm, ctx->stream));
-      std::swap(u_old, u_cur);
+      std::swap(ub[0], ub[1]);
+      k--;
     }
   }
   HIPCHK(born::launch_adj_finish(geo, img, ctx->d_acc, ctx->d_vel2,
@@ -1353,7 +1401,9 @@ int born_set_step_pairs(born_ctx *ctx, int mode) {
   return BORN_OK;
 }
 
-int born_last_step_pairs(const born_ctx *ctx) { return ctx ? ctx->last_pairs : 0; }
+int born_last_step_pairs(const born_ctx *ctx) {
+  return ctx ? ctx->last_pairs | ctx->last_lam_pairs << 1 : 0;
+}
 
 int born_set_storage(born_ctx *ctx, int storage) {
   if (!ctx) return BORN_ERR_ARG;

@@ -216,6 +216,18 @@ __device__ __forceinline__ float add_points(const Points &pts, uint32_t m, int x
   return v;
 }
 
+// Receiver injection of the adjoint-state step: u + V b g, g the sum over
+// the receivers at (x, y) of their time-sinc interpolated residual
+// (born_abi.hip, exact adjoint).  Rounding fixed as in leap().
+__device__ __forceinline__ float lam_inject(const RecMap &rm, int slot, int x, int y, int nx,
+                                            int ny, const float *tr, const float *tab, int id,
+                                            float v, float b, float u) {
+  float g = 0.f;
+  for (int i = rm.rmap[((long long)slot * ny + y) * nx + x]; i >= 0; i = rm.rnext[i])
+    g = add_r(g, sinc8(tab, tr + (long long)id * rm.nrec + i, rm.nrec));
+  return fma_r(mul_r(v, b), g, u);
+}
+
 struct CellArgs {
   int x, y, z;
   uint32_t o;  // element offset of the cell

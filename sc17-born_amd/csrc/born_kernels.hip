@@ -23,6 +23,7 @@
 #include "born_device.h"
 #include "born_launch.h"
 #include "born_pair.h"
+#include "born_pair_adj.h"
 
 namespace born {
 
@@ -424,15 +425,10 @@ struct LamOp {
                        float (&tmax)[NF]) const {
     const float bb = c.b, v = p[1];
     float un = leap(bb, v, lap[0], ctr[0], p[0]);
-    if (slot >= 0 && inject) {
-      float g = 0.f;
-      for (int i = rec_first(rm, slot, c.x, c.y, nx, ny); i >= 0; i = rm.rnext[i])
-        g += sinc8(tab, tr + (long long)id * rm.nrec + i, rm.nrec);
-      un += v * bb * g;
-    }
+    if (slot >= 0 && inject) un = lam_inject(rm, slot, c.x, c.y, nx, ny, tr, tab, id, v, bb, un);
     if constexpr (H) tmax[0] = fmaxf(tmax[0], fabsf(un));
     out[0] = un;
-    out[1] = p[3] + p[2] * un;
+    out[1] = fma_r(p[2], un, p[3]);
   }
 };
 
@@ -696,6 +692,34 @@ hipError_t launch_pair_step(const Geometry &geo, const Tune &tu, const PairStep 
   }
   PairOp<1> op{};
   return fill(op);
+}
+
+template <int RY, int WY>
+static hipError_t launch_lam_pair_shape(const Geometry &geo, int zc, const LamPairOp &op,
+                                        hipStream_t st) {
+  const Grid g = pair_grid(geo.nx, geo.ny, geo.nz, RY * WY, zc > 0 ? zc : 64);
+  const int nb = g.ntx * g.nty * g.ntz;
+  hipLaunchKernelGGL((lam_pair_kernel<RY, WY>), dim3(nb), dim3(kTX * WY), 0, st, g, geo.taps,
+                     geo.sponge, op);
+  return hipGetLastError();
+}
+
+hipError_t launch_lam_pair_step(const Geometry &geo, const Tune &tu, const LamPairStep &a,
+                                hipStream_t st) {
+  if (geo.nx < 2 * kH + 1 || geo.ny < 2 * kH + 1 || geo.nz < 2 * kH + 1)
+    return hipErrorInvalidValue;
+  LamPairOp op{};
+  op.u_old = a.u_old; op.u_cur = a.u_cur; op.u_a = a.u_a; op.u_b = a.u_b;
+  op.vel = a.vel; op.sig_a = a.sig_a; op.sig_b = a.sig_b; op.acc = a.acc;
+  op.rm = a.rec; op.tr = a.tr; op.tab_a = a.tab_a; op.tab_b = a.tab_b;
+  op.id_a = a.id_a; op.id_b = a.id_b; op.inj_a = a.inj_a; op.inj_b = a.inj_b;
+  switch (tu.shape) {
+    case 0: return launch_lam_pair_shape<2, 8>(geo, tu.zchunk, op, st);
+    case 1: return launch_lam_pair_shape<1, 8>(geo, tu.zchunk, op, st);
+    case 2: return launch_lam_pair_shape<2, 4>(geo, tu.zchunk, op, st);
+    case 3: return launch_lam_pair_shape<4, 4>(geo, tu.zchunk, op, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_src_step(const Geometry &geo, const Tune &tu, const SrcStep &a,

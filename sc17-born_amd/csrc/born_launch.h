@@ -77,6 +77,19 @@ struct PairStep {
   int id_a, id_b, ext_a, ext_b;
   int scatter;
 };
+// Two adjoint-state steps in one launch (FP32): steps k and k-1 of the exact
+// adjoint; nu levels j-1, j read, j+1 (u_a) and j+2 (u_b) written; four
+// distinct buffers.  *_a belong to step k, *_b to step k-1.
+struct LamPairStep {
+  const float *u_old, *u_cur;
+  float *u_a, *u_b;
+  const float *vel, *sig_a, *sig_b;
+  float *acc;
+  RecMap rec;
+  const float *tr;
+  const float *tab_a, *tab_b;
+  int id_a, id_b, inj_a, inj_b;
+};
 // Pair kernel tile: RY tile rows per thread, WY wavefronts (56 x RY*WY cells).
 struct PairShape {
   int ry, wy;
@@ -171,6 +184,8 @@ hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a
                            hipStream_t st);
 hipError_t launch_pair_step(const Geometry &geo, const Tune &tu, const PairStep &a,
                             hipStream_t st);
+hipError_t launch_lam_pair_step(const Geometry &geo, const Tune &tu, const LamPairStep &a,
+                                hipStream_t st);
 hipError_t launch_batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
                                  hipStream_t st);
 hipError_t launch_batch_ring_init(float *rings, int nshots, hipStream_t st);

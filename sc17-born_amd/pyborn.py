@@ -233,7 +233,8 @@ class Propagator:
 
     KERNELS = ("fwd", "fwd_dV", "source", "rtm_adjoint", "background", "background_dV",
                "adjoint_state", "fwd_f16", "fwd_dV_f16", "background_f16",
-               "background_dV_f16", "adjoint_state_f16", "fwd_pair", "fwd_dV_pair")
+               "background_dV_f16", "adjoint_state_f16", "fwd_pair", "fwd_dV_pair",
+               "adjoint_state_pair")
     STORAGE = {"f32": 0, "f16": 1}
 
     def set_storage(self, kind):
@@ -249,7 +250,13 @@ class Propagator:
 
     @property
     def last_step_pairs(self):
-        return bool(self.lib.born_last_step_pairs(self.h))
+        """True when the last Born forward ran in two-step launches."""
+        return bool(self.lib.born_last_step_pairs(self.h) & 1)
+
+    @property
+    def last_adjoint_pairs(self):
+        """True when the last exact adjoint ran in two-step launches."""
+        return bool(self.lib.born_last_step_pairs(self.h) & 2)
 
     def tuning(self):
         out = {}

@@ -135,3 +135,65 @@ def test_kept_history_gives_the_same_exact_adjoint(pyborn, scatter):
         imgs.append(img.cpu().numpy())
         H.close()
     _check(imgs[0], imgs[1], "image from kept history, scatter %d" % scatter)
+
+
+# ---- adjoint-state sweep of the exact adjoint in two-step launches
+# (lam_pair_kernel, born_pair_adj.h)
+
+def _adjoint(pyborn, c, w, scatter, pairs, shape=None, zc=None, budget=None):
+    H = hip_prop(pyborn, c)
+    H.set_step_pairs(pairs)
+    if shape is not None:
+        H.set_tuning("adjoint_state_pair", shape[0], shape[1], 1, 1, zc)
+    if budget is not None:
+        H.set_checkpoint_budget(budget)
+    H.transfer_receiver_func(c.rec_nx, c.rec_ny, c.rec_locs, np.zeros((c.nrec, c.ntb), np.float32))
+    g = np.zeros(c.nx * c.ny * c.nz, np.float32)
+    H.adjoint_exact(c.jt, w, g, c.nt, scatter=scatter)
+    used = H.last_adjoint_pairs
+    plan = H.last_checkpoint_plan()
+    H.close()
+    return g, used, plan
+
+
+def _residual(c, seed=7):
+    return np.random.default_rng(seed).standard_normal((c.nrec, c.ntb)).astype(np.float32)
+
+
+@pytest.mark.parametrize("scatter", [0, 1])
+def test_adjoint_pairs_match_single_steps_and_oracle(pyborn, scatter):
+    c = make_case(120, 40, 44, nt=91, nbound=10, src_z=12)
+    w = _residual(c)
+    got, used, _ = _adjoint(pyborn, c, w, scatter, True)
+    assert used
+    want, used1, _ = _adjoint(pyborn, c, w, scatter, False)
+    assert not used1
+    _check(got, want, "adjoint 120x40x44 scatter %d" % scatter)
+    P = oracle_prop(c)
+    P.transfer_receiver_func(c.rec_locs, np.zeros((c.nrec, c.ntb), np.float32))
+    g64 = P.exact_adjoint_d(c.jt, w.astype(np.float64), c.nt, scatter=scatter)
+    assert rel_l2(got, g64.ravel()) <= 1e-4
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("zc", [8, 13, 200])
+def test_every_adjoint_pair_shape_and_z_chunk(pyborn, shape, zc):
+    # receivers inject on a plane next to chunk edges; jt = 3 sinc rows
+    c = make_case(121, 43, 37, nt=40, nbound=8, src_z=9, rec_z=7, seed=3, jt=3)
+    w = _residual(c, 11)
+    want, _, _ = _adjoint(pyborn, c, w, 0, False)
+    got, used, _ = _adjoint(pyborn, c, w, 0, True, shape=shape, zc=zc)
+    assert used
+    _check(got, want, "adjoint shape %s zc %d" % (shape, zc))
+
+
+def test_adjoint_pairs_across_checkpoint_segments(pyborn):
+    # a small budget: several segments of odd length, each swept in pairs
+    # plus a single step
+    c = make_case(64, 40, 36, nt=67, nbound=8)
+    w = _residual(c, 5)
+    n = c.nx * c.ny * c.nz * 4
+    want, _, plan0 = _adjoint(pyborn, c, w, 1, False, budget=26 * n)
+    got, used, plan = _adjoint(pyborn, c, w, 1, True, budget=26 * n)
+    assert used and plan == plan0 and plan[0] > 2
+    _check(got, want, "adjoint, %d checkpoint segments" % plan[0])
