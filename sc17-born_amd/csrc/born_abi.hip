@@ -924,8 +924,10 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   // With scatter 0 the scatter operand of step it is s of level it+1, so a
   // kept history holds the s levels: step it reads levels it (history it-1)
   // and it-1 (history it-2) and writes level it+1 into history it, one
-  // write per cell less than a separate s buffer and history.
-  const bool s_in_hist = hist && scatter == 0;
+  // write per cell less than a separate s buffer and history.  (FP16 levels
+  // keep s and r together as one __half2 per cell in sb[], and the history
+  // is written apart.)
+  const bool s_in_hist = hist && scatter == 0 && !half;
   for (; it < nt; it++) {
     const int id = it / jt, ii = it - id * jt;
     if (s_in_hist) {

@@ -234,6 +234,44 @@ struct CellArgs {
   float b;     // sponge factor (1 outside the sponge)
 };
 
+__device__ __forceinline__ float sat_half(float v) { return fminf(fmaxf(v, -65504.f), 65504.f); }
+
+// ---- Two FP16 fields stored as one __half2 per cell (first field in the
+// low half): one 4-byte load brings both, half the load instructions of two
+// separate FP16 arrays.  inv = 1/scale of each field.
+template <int VX>
+__device__ __forceinline__ void ld_pair(const __half2 *p, uint32_t o, const float (&inv)[2],
+                                        float (&v)[2][VX]) {
+  if constexpr (VX == 1) {
+    const __half2 w = at(p, o);
+    v[0][0] = __low2float(w) * inv[0];
+    v[1][0] = __high2float(w) * inv[1];
+  } else {
+    const uint2 u = at((const uint2 *)p, o >> 1);
+    __half2 a, b;
+    __builtin_memcpy(&a, &u.x, 4);
+    __builtin_memcpy(&b, &u.y, 4);
+    v[0][0] = __low2float(a) * inv[0];
+    v[1][0] = __high2float(a) * inv[1];
+    v[0][1] = __low2float(b) * inv[0];
+    v[1][1] = __high2float(b) * inv[1];
+  }
+}
+template <int VX>
+__device__ __forceinline__ void st_pair(__half2 *p, uint32_t o, const float (&a)[VX],
+                                        const float (&b)[VX], const float (&scale)[2]) {
+  if constexpr (VX == 1) {
+    at(p, o) = __floats2half2_rn(sat_half(a[0] * scale[0]), sat_half(b[0] * scale[1]));
+  } else {
+    const __half2 c0 = __floats2half2_rn(sat_half(a[0] * scale[0]), sat_half(b[0] * scale[1]));
+    const __half2 c1 = __floats2half2_rn(sat_half(a[1] * scale[0]), sat_half(b[1] * scale[1]));
+    uint2 u;
+    __builtin_memcpy(&u.x, &c0, 4);
+    __builtin_memcpy(&u.y, &c1, 4);
+    at((uint2 *)p, o >> 1) = u;
+  }
+}
+
 // ---- VX consecutive cells of a row (VX = 1 or 2).  A pair starts at an
 // even element offset (x tiles start at even x and nx is even when VX = 2),
 // so float2 / __half2 accesses are naturally aligned.
@@ -265,7 +303,6 @@ __device__ __forceinline__ void st_vec(float *p, uint32_t o, const float (&v)[VX
   else
     at((float2 *)p, o >> 1) = make_float2(v[0], v[1]);
 }
-__device__ __forceinline__ float sat_half(float v) { return fminf(fmaxf(v, -65504.f), 65504.f); }
 template <int VX>
 __device__ __forceinline__ void st_vec(__half *p, uint32_t o, const float (&v)[VX],
                                        float scale) {
@@ -282,6 +319,8 @@ __device__ __forceinline__ void st_vec(__half *p, uint32_t o, const float (&v)[V
 //   Blk block(x0, x1, y0, y1, z0, z1);          uniform per-block state
 //   void first(const Blk&);                     (NMAX > 0) once per launch
 //   ldv<VX>(const Blk&, int f, uint32_t o, float (&v)[VX])   current level
+//   (PAIRED ops instead: ldv2<VX>(const Blk&, uint32_t o, float (&v)[NF][VX])
+//   loads every field of the current level at once)
 //   Pln plane(const Blk&, int z);               uniform per-plane state
 //   prefetchv<VX>(const Blk&, float (&p)[NPRE][VX], uint32_t o)
 //   void cell(const Blk&, const Pln&, const CellArgs&, const float *lap,
@@ -400,36 +439,52 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   };
   float pre[PZ][RY][NP][VX], pre_n[PZ][RY][NP][VX];
   float tmax[NF];
+  // every field of the current level at element offset o
+  auto ld_all = [&](uint32_t o, float (&v)[NF][VX]) {
+    if constexpr (Op::PAIRED) {
+      op.template ldv2<VX>(blk, o, v);
+    } else {
 #pragma unroll
-  for (int f = 0; f < NF; f++) {
-    tmax[f] = 0.f;
+      for (int f = 0; f < NF; f++) op.template ldv<VX>(blk, f, o, v[f]);
+    }
+  };
 #pragma unroll
-    for (int j = 0; j < RY; j++) {
+  for (int f = 0; f < NF; f++) tmax[f] = 0.f;
 #pragma unroll
-      for (int k = 0; k < 2 * kH; k++) {
-        float w[VX];
-        op.template ldv<VX>(blk, f, (uint32_t)(z0 - kH + k) * g.plane + row[j], w);
+  for (int j = 0; j < RY; j++) {
 #pragma unroll
-        for (int v = 0; v < VX; v++) q[f][j][v][k] = w[v];
-      }
+    for (int k = 0; k < 2 * kH; k++) {
+      float w[NF][VX];
+      ld_all((uint32_t)(z0 - kH + k) * g.plane + row[j], w);
 #pragma unroll
-      for (int p = 0; p < PZ; p++)
-        op.template ldv<VX>(blk, f, (uint32_t)min(z0 + kH + p, g.nz - 1) * g.plane + row[j],
-                            lead[p][f][j]);
+      for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int v = 0; v < VX; v++) q[f][j][v][k] = w[f][v];
     }
 #pragma unroll
-    for (int p = 0; p < kH + PZ; p++) {
-      const int zp = min(z0 + p, g.nz - 1);
+    for (int p = 0; p < PZ; p++) {
+      float w[NF][VX];
+      ld_all((uint32_t)min(z0 + kH + p, g.nz - 1) * g.plane + row[j], w);
 #pragma unroll
-      for (int k = 0; k < HPT; k++) {
+      for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int v = 0; v < VX; v++) lead[p][f][j][v] = w[f][v];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < kH + PZ; p++) {
+    const int zp = min(z0 + p, g.nz - 1);
+#pragma unroll
+    for (int k = 0; k < HPT; k++) {
+      float h[NF][VX];
+      ld_all((uint32_t)zp * g.plane + hoff[k], h);
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
         if (p < kH) {
-          if (hv[k]) {
-            float h[VX];
-            op.template ldv<VX>(blk, f, (uint32_t)zp * g.plane + hoff[k], h);
-            put_halo((z0 + p) % NR, f, k, h);
-          }
+          if (hv[k]) put_halo((z0 + p) % NR, f, k, h[f]);
         } else {
-          op.template ldv<VX>(blk, f, (uint32_t)zp * g.plane + hoff[k], hl[p - kH][f][k]);
+#pragma unroll
+          for (int v = 0; v < VX; v++) hl[p - kH][f][k][v] = h[f][v];
         }
       }
     }
@@ -493,11 +548,22 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
         const uint32_t zl = (uint32_t)min(z + PZ + kH + p, g.nz - 1) * g.plane;
         const uint32_t zn = (uint32_t)min(z + PZ + p, g.nz - 1) * g.plane;
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
+        for (int j = 0; j < RY; j++) {
+          float w[NF][VX];
+          ld_all(zl + row[j], w);
 #pragma unroll
-          for (int j = 0; j < RY; j++) op.template ldv<VX>(blk, f, zl + row[j], lead[p][f][j]);
+          for (int f = 0; f < NF; f++)
 #pragma unroll
-          for (int k = 0; k < HPT; k++) op.template ldv<VX>(blk, f, zl + hoff[k], hl[p][f][k]);
+            for (int v = 0; v < VX; v++) lead[p][f][j][v] = w[f][v];
+        }
+#pragma unroll
+        for (int k = 0; k < HPT; k++) {
+          float w[NF][VX];
+          ld_all(zl + hoff[k], w);
+#pragma unroll
+          for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int v = 0; v < VX; v++) hl[p][f][k][v] = w[f][v];
         }
 #pragma unroll
         for (int j = 0; j < RY; j++) op.template prefetchv<VX>(blk, pre_n[p][j], zn + row[j]);

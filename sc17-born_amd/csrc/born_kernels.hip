@@ -67,6 +67,9 @@ template <int SCATTER, class T>
 struct FwdOp {
   static constexpr bool H = kHalf<T>;
   static constexpr int NF = 2, NPRE = 4, NOUT = 3, NMAX = H ? 2 : 0;
+  // FP16 levels hold s and r as one __half2 per cell (sr1: level k, sr0:
+  // level k-1, srn: level k+1, the same buffer as sr0 in place)
+  static constexpr bool PAIRED = H;
   static constexpr bool DAMP = true;
   const T *__restrict__ s1;
   const T *__restrict__ r1;
@@ -76,6 +79,9 @@ struct FwdOp {
   const T *s0;
   T *s_new;
   T *__restrict__ r0;
+  const __half2 *sr1;
+  const __half2 *sr0;
+  __half2 *srn;
   const float *__restrict__ vel;
   const float *__restrict__ model;
   T *__restrict__ hist;  // optional scatter-operand history
@@ -135,20 +141,38 @@ struct FwdOp {
   __device__ void ldv(const Blk2 &b, int f, uint32_t o, float (&v)[VX]) const {
     ld_vec<VX>(f == 0 ? s1 : r1, o, b.ic[f], v);
   }
+  template <int VX>
+  __device__ void ldv2(const Blk2 &b, uint32_t o, float (&v)[2][VX]) const {
+    ld_pair<VX>(sr1, o, b.ic, v);
+  }
   __device__ SrcPln plane(const Blk2 &b, int z) const {
     return {points_on_plane(pts, b.pm, z), ldc(rm.plane_slot, z)};
   }
   template <int VX>
   __device__ void prefetchv(const Blk2 &b, float (&p)[NPRE][VX], uint32_t o) const {
-    ld_vec<VX>(s0, o, b.io[0], p[0]);
-    ld_vec<VX>(r0, o, b.io[1], p[1]);
+    if constexpr (H) {
+      float sr[2][VX];
+      ld_pair<VX>(sr0, o, b.io, sr);
+#pragma unroll
+      for (int v = 0; v < VX; v++) {
+        p[0][v] = sr[0][v];
+        p[1][v] = sr[1][v];
+      }
+    } else {
+      ld_vec<VX>(s0, o, b.io[0], p[0]);
+      ld_vec<VX>(r0, o, b.io[1], p[1]);
+    }
     ld_vec<VX>(vel, o, 1.f, p[2]);
     ld_vec<VX>(model, o, 1.f, p[3]);
   }
   template <int VX>
   __device__ void storev(const Blk2 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
-    st_vec<VX>(s_new, o, out[0], b.sn[0]);
-    st_vec<VX>(r0, o, out[1], b.sn[1]);
+    if constexpr (H) {
+      st_pair<VX>(srn, o, out[0], out[1], b.sn);
+    } else {
+      st_vec<VX>(s_new, o, out[0], b.sn[0]);
+      st_vec<VX>(r0, o, out[1], b.sn[1]);
+    }
     if (hist) st_vec<VX>(hist, o, out[2], b.sh);
   }
   __device__ void cell(const Blk2 &b, const SrcPln &pl, const CellArgs &c,
@@ -186,6 +210,7 @@ struct FwdOp {
 // ------------------------------------------------------------- sourceProp
 struct SrcOp {
   static constexpr int NF = 1, NPRE = 2, NOUT = 1, NMAX = 0;
+  static constexpr bool PAIRED = false;
   static constexpr bool DAMP = false;
   const float *__restrict__ s1;
   float *__restrict__ s0;
@@ -223,6 +248,7 @@ struct SrcOp {
 // ------------------------------------------------------ reference adjoint
 struct RadjOp {
   static constexpr int NF = 2, NPRE = 5, NOUT = 3, NMAX = 0;
+  static constexpr bool PAIRED = false;
   static constexpr bool DAMP = true;
   const float *__restrict__ s1;
   const float *__restrict__ r1;
@@ -295,6 +321,7 @@ template <int SCATTER, class T>
 struct SrecOp {
   static constexpr bool H = kHalf<T>;
   static constexpr int NF = 1, NPRE = 2, NOUT = 2, NMAX = H ? 1 : 0;
+  static constexpr bool PAIRED = false;
   static constexpr bool DAMP = true;
   const T *__restrict__ s1;
   T *__restrict__ s0;
@@ -365,6 +392,7 @@ template <class T>
 struct LamOp {
   static constexpr bool H = kHalf<T>;
   static constexpr int NF = 1, NPRE = 4, NOUT = 2, NMAX = H ? 1 : 0;
+  static constexpr bool PAIRED = false;
   static constexpr bool DAMP = true;
   const T *__restrict__ u1;
   T *__restrict__ u0;
@@ -462,6 +490,9 @@ struct BatchFwd {
     o.s0 = base.s0 + d.cell0;
     o.s_new = base.s_new + d.cell0;
     o.r0 = base.r0 + d.cell0;
+    o.sr1 = base.sr1 + d.cell0;
+    o.sr0 = base.sr0 + d.cell0;
+    o.srn = base.srn + d.cell0;
     o.vel = base.vel + d.cell0;
     o.model = base.model + d.cell0;
     o.hist = nullptr;
@@ -625,6 +656,8 @@ hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a
     using T = std::remove_pointer_t<decltype(tp)>;
     op.s1 = as<T>(a.s1); op.r1 = as<T>(a.r1); op.s0 = as<T>(a.s0); op.r0 = as<T>(a.r0);
     op.s_new = as<T>(a.s_new ? a.s_new : a.s0);
+    op.sr1 = (const __half2 *)a.s1; op.sr0 = (const __half2 *)a.s0;
+    op.srn = (__half2 *)(a.s_new ? a.s_new : a.s0);
     op.vel = a.vel; op.model = a.model; op.hist = as<T>(a.hist);
     op.pts = to_points(a.src); op.amp = a.amp;
     op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
@@ -865,6 +898,7 @@ hipError_t launch_batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFw
     auto &op = bop.base;
     op.s1 = as<T>(a.s1); op.r1 = as<T>(a.r1); op.s0 = as<T>(a.s0); op.r0 = as<T>(a.r0);
     op.s_new = as<T>(a.s0);
+    op.sr1 = (const __half2 *)a.s1; op.sr0 = (const __half2 *)a.s0; op.srn = (__half2 *)a.s0;
     op.vel = a.vel; op.model = a.model; op.hist = nullptr;
     op.pts = to_points(a.src); op.amp = a.amp;
     op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
