@@ -30,14 +30,15 @@ PKG = os.path.join(ROOT, "sc17-born_amd")
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per interior cell and step, FP32 / FP16 levels
 # (vel, model, image and traces stay FP32):
-#   forward: s, r read cur + old, vel, model; s, r written (s into the kept
-#     history with scatter 0)                                   32 / 20
+#   forward: s, r read cur + old, vel, model; s, r written (FP32: s into
+#     the kept history with scatter 0; FP16: s and r as one __half2 per cell
+#     and the history written apart)                            32 / 22
 #   adjoint state: nu read cur + old, vel, sigma, image; nu and image
 #     written                                                   28 / 20
-#   background recompute (steps of checkpoint segments): s read cur + old,
-#     vel; s and sigma written                                  20 / 12
+#   background recompute (steps of checkpoint segments), scatter 0: s read
+#     cur + old, vel; s written (into the segment history)      16 / 10
 FWD_BYTES_PER_CELL = 32
-BYTES = {"f32": {"fwd": 32, "lam": 28, "srec": 20}, "f16": {"fwd": 20, "lam": 20, "srec": 12}}
+BYTES = {"f32": {"fwd": 32, "lam": 28, "srec": 16}, "f16": {"fwd": 22, "lam": 20, "srec": 10}}
 
 
 def born_bytes(storage, nt, recomputed):

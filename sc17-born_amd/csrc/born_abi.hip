@@ -1143,6 +1143,9 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   }
   CHK(source_amps(ctx, nt, jt, 0, 1 / (float)jt));
   float *s_old = ctx->d_w[2], *s_cur = ctx->d_w[3];
+  // scatter 0: sigma of step k is s of level k+1, so the background steps
+  // that fill the segment history write their s levels straight into it
+  const bool s_in_seg = scatter == 0;
   born::SrecStep sr{};
   sr.vel = ctx->d_vel2;
   sr.src = points(ctx);
@@ -1168,14 +1171,27 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
           HIPCHK(hipMemcpyAsync(ctx->d_ckring + 8 * c, ctx->d_ring + RING_S, 8 * sizeof(float),
                                 hipMemcpyDeviceToDevice, ctx->stream));
       }
-      sr.s1 = s_cur;
-      sr.s0 = s_old;
-      sr.sig = k >= klast ? nth(ctx->d_seg, k - klast, n, esz) : nullptr;
-      sr.hscale = half && sr.sig ? ctx->d_hscale + (k - klast) : nullptr;
+      float *sig = k >= klast ? nth(ctx->d_seg, k - klast, n, esz) : nullptr;
+      const bool into_seg = s_in_seg && k >= klast;
+      if (into_seg) {
+        // levels klast and klast-1 stay in s_cur and s_old; level j > klast
+        // is in segment level j-1-klast
+        sr.s1 = k > klast ? nth(ctx->d_seg, k - 1 - klast, n, esz) : s_cur;
+        sr.s0 = k > klast + 1 ? nth(ctx->d_seg, k - 2 - klast, n, esz)
+                              : (k > klast ? s_cur : s_old);
+        sr.s_new = sig;
+        sr.sig = nullptr;
+      } else {
+        sr.s1 = s_cur;
+        sr.s0 = s_old;
+        sr.s_new = nullptr;
+        sr.sig = sig;
+      }
+      sr.hscale = half && sig ? ctx->d_hscale + (k - klast) : nullptr;
       sr.amp = ctx->d_amp + (long long)k * ctx->npts_s;
       sr.h = half_state(ctx, k);
       HIPCHK(born::launch_srec_step(geo, stu, sr, ctx->stream));
-      std::swap(s_old, s_cur);
+      if (!into_seg) std::swap(s_old, s_cur);
     }
   }
   // pass 2: adjoint state, segment by segment in reverse
@@ -1224,14 +1240,23 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
                                 hipMemcpyDeviceToDevice, ctx->stream));
       }
       for (int k = k0; k < k1; k++) {
-        sr.s1 = s_cur;
-        sr.s0 = s_old;
-        sr.sig = nth(ctx->d_seg, k - k0, n, esz);
+        float *sig = nth(ctx->d_seg, k - k0, n, esz);
+        if (s_in_seg) {
+          sr.s1 = k > k0 ? nth(ctx->d_seg, k - 1 - k0, n, esz) : s_cur;
+          sr.s0 = k > k0 + 1 ? nth(ctx->d_seg, k - 2 - k0, n, esz) : (k > k0 ? s_cur : s_old);
+          sr.s_new = sig;
+          sr.sig = nullptr;
+        } else {
+          sr.s1 = s_cur;
+          sr.s0 = s_old;
+          sr.s_new = nullptr;
+          sr.sig = sig;
+        }
         sr.hscale = half ? ctx->d_hscale + (k - k0) : nullptr;
         sr.amp = ctx->d_amp + (long long)k * ctx->npts_s;
         sr.h = half_state(ctx, k);
         HIPCHK(born::launch_srec_step(geo, stu, sr, ctx->stream));
-        std::swap(s_old, s_cur);
+        if (!s_in_seg) std::swap(s_old, s_cur);
         recomputed++;
       }
     }

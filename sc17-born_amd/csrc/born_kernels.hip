@@ -324,7 +324,11 @@ struct SrecOp {
   static constexpr bool PAIRED = false;
   static constexpr bool DAMP = true;
   const T *__restrict__ s1;
-  T *__restrict__ s0;
+  // old s level (read) and new s level (written): the same buffer, or with
+  // scatter 0 two levels of the segment history (level k of s is sigma of
+  // step k-1)
+  const T *s0;
+  T *s_new;
   const float *__restrict__ vel;
   T *__restrict__ sig;  // scatter operand of this step (nullptr: none)
   Points pts;
@@ -351,7 +355,7 @@ struct SrecOp {
   __device__ void first(const Blk1 &b) const {
     ring.scale[sl.nxt] = b.sn;
     ring.maxb[sl.clr] = 0u;
-    if (sig) *hscale = b.sh;
+    if (hscale) *hscale = b.sh;
   }
   __device__ void publish_max(int, float m) const {
     atomicMax(&ring.maxb[sl.nxt], __float_as_uint(m));
@@ -370,7 +374,7 @@ struct SrecOp {
   }
   template <int VX>
   __device__ void storev(const Blk1 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
-    st_vec<VX>(s0, o, out[0], b.sn);
+    st_vec<VX>(s_new, o, out[0], b.sn);
     if (sig) st_vec<VX>(sig, o, out[1], b.sh);
   }
   __device__ void cell(const Blk1 &, uint32_t pm, const CellArgs &c, const float *lap,
@@ -780,6 +784,7 @@ hipError_t launch_srec_step(const Geometry &geo, const Tune &tu, const SrecStep 
   auto fill = [&](auto &op, auto *tp) {
     using T = std::remove_pointer_t<decltype(tp)>;
     op.s1 = as<T>(a.s1); op.s0 = as<T>(a.s0); op.vel = a.vel; op.sig = as<T>(a.sig);
+    op.s_new = as<T>(a.s_new ? a.s_new : a.s0);
     op.pts = to_points(a.src); op.amp = a.amp;
     op.ring = a.ring; op.sl = a.h.sl; op.gain = a.h.gain; op.tapsum = a.h.tapsum;
     op.hscale = a.hscale;

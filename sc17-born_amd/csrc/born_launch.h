@@ -124,6 +124,7 @@ struct RadjStep {
 struct SrecStep {
   const float *s1;
   float *s0;
+  float *s_new;  // where the new s level goes (null: s0, in place)
   const float *vel;
   float *sig;
   SrcPts src;

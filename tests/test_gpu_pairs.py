@@ -197,3 +197,32 @@ def test_adjoint_pairs_across_checkpoint_segments(pyborn):
     got, used, plan = _adjoint(pyborn, c, w, 1, True, budget=26 * n)
     assert used and plan == plan0 and plan[0] > 2
     _check(got, want, "adjoint, %d checkpoint segments" % plan[0])
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("scatter", [0, 1])
+def test_adjoint_recompute_matches_kept_history(pyborn, storage, scatter):
+    # the background history the forward keeps and the one the exact adjoint
+    # recomputes (with scatter 0 both write s straight into the history) give
+    # the same image bit for bit
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    c = make_case(70, 44, 40, nt=53, nbound=8)
+    rng = np.random.default_rng(9)
+    w = torch.from_numpy(rng.standard_normal((c.ntb, c.nrec)).astype(np.float32)).to(dev)
+    model = torch.from_numpy(c.model.ravel()).to(dev)
+    imgs = []
+    for keep in (True, False):
+        H = hip_prop(pyborn, c)
+        H.set_storage(storage)
+        H.transfer_receiver_func(c.rec_nx, c.rec_ny, c.rec_locs,
+                                 np.zeros((c.nrec, c.ntb), np.float32))
+        if keep:
+            tr = torch.zeros((c.ntb, c.nrec), dtype=torch.float32, device=dev)
+            H.dev_forward(c.jt, c.nt, model, tr, scatter=scatter, keep_background=True)
+        img = torch.zeros(c.nx * c.ny * c.nz, dtype=torch.float32, device=dev)
+        H.dev_adjoint_exact(c.jt, c.nt, w, img, scatter=scatter)
+        H.synchronize()
+        imgs.append(img.cpu().numpy())
+        H.close()
+    _check(imgs[1], imgs[0], "recomputed vs kept history, %s scatter %d" % (storage, scatter))
