@@ -374,7 +374,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   float bxy[RY][VX];
 #pragma unroll
   for (int j = 0; j < RY; j++) {
-    yy[j] = y0 + ty + NG * j;
+    yy[j] = y0 + RY * ty + j;  // a thread's rows are consecutive
     in[j] = xin && yy[j] < g.ny - kH;
     const int yc = min(yy[j], g.ny - 1);
     row[j] = (uint32_t)yc * g.nx + xc;
@@ -526,7 +526,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
           for (int v = 0; v < VX; v++) q[f][j][v][2 * kH + p] = lead[p][f][j][v];
 #pragma unroll
         for (int p = 0; p < PZ; p++) {
-          float *d = &tile[(z + p) % NR][f][kH + ty + NG * j][kH + VX * tx];
+          float *d = &tile[(z + p) % NR][f][kH + RY * ty + j][kH + VX * tx];
           if constexpr (VX == 1)
             d[0] = q[f][j][0][kH + p];
           else
@@ -578,39 +578,49 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
       const uint32_t zo = (uint32_t)zz * g.plane;
       const auto pl = op.plane(blk, zz);
       const float bzv = Op::DAMP ? ldc(sp.bz, zz) : 1.f;
+      // Laplacians field by field.  The column above and below this
+      // thread's RY consecutive rows is read from LDS once per field (kH
+      // rows either side; the thread's own rows come from its z queues) and
+      // shared by the RY cells.
+      const int lr0 = kH + RY * ty, lc = kH + VX * tx;
+      float lap[RY][VX][NF];
 #pragma unroll
-      for (int j = 0; j < RY; j++) {
-        if (!in[j]) continue;
-        const int lr = kH + ty + NG * j, lc = kH + VX * tx;
-        float lap[VX][NF], ctr[VX][NF];
+      for (int f = 0; f < NF; f++) {
+        float col[RY + 2 * kH][VX];
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
-          // the row from lc - 4 to lc + VX + 3, and the columns above/below
+        for (int i = 0; i < kH; i++) {
+          const int ra = lr0 - kH + i, rb = lr0 + RY + i;
+          if constexpr (VX == 1) {
+            col[i][0] = tile[sl][f][ra][lc];
+            col[kH + RY + i][0] = tile[sl][f][rb][lc];
+          } else {
+            const float2 a2 = *(const float2 *)&tile[sl][f][ra][lc];
+            const float2 b2 = *(const float2 *)&tile[sl][f][rb][lc];
+            col[i][0] = a2.x;
+            col[i][1] = a2.y;
+            col[kH + RY + i][0] = b2.x;
+            col[kH + RY + i][1] = b2.y;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < RY; j++)
+#pragma unroll
+          for (int v = 0; v < VX; v++) col[kH + j][v] = q[f][j][v][kH + p];
+#pragma unroll
+        for (int j = 0; j < RY; j++) {
+          if (!in[j]) continue;
+          // the row from lc - 4 to lc + VX + 3
+          const int lr = lr0 + j;
           float xs[2 * kH + VX];
-          float up[kH][VX], dn[kH][VX];
           if constexpr (VX == 1) {
 #pragma unroll
             for (int i = 0; i < 2 * kH + 1; i++) xs[i] = tile[sl][f][lr][lc - kH + i];
-#pragma unroll
-            for (int k = 1; k <= kH; k++) {
-              up[k - 1][0] = tile[sl][f][lr - k][lc];
-              dn[k - 1][0] = tile[sl][f][lr + k][lc];
-            }
           } else {
 #pragma unroll
             for (int i = 0; i < kH + 1; i++) {
               const float2 w = *(const float2 *)&tile[sl][f][lr][lc - kH + 2 * i];
               xs[2 * i] = w.x;
               xs[2 * i + 1] = w.y;
-            }
-#pragma unroll
-            for (int k = 1; k <= kH; k++) {
-              const float2 a = *(const float2 *)&tile[sl][f][lr - k][lc];
-              const float2 b = *(const float2 *)&tile[sl][f][lr + k][lc];
-              up[k - 1][0] = a.x;
-              up[k - 1][1] = a.y;
-              dn[k - 1][0] = b.x;
-              dn[k - 1][1] = b.y;
             }
           }
 #pragma unroll
@@ -622,14 +632,22 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
               a = fma_r(t.cx[k - 1], add_r(xs[kH + v - k], xs[kH + v + k]), a);
 #pragma unroll
             for (int k = 1; k <= kH; k++)
-              a = fma_r(t.cy[k - 1], add_r(up[k - 1][v], dn[k - 1][v]), a);
+              a = fma_r(t.cy[k - 1], add_r(col[kH + j - k][v], col[kH + j + k][v]), a);
 #pragma unroll
             for (int k = 1; k <= kH; k++)
               a = fma_r(t.cz[k - 1], add_r(q[f][j][v][kH + p - k], q[f][j][v][kH + p + k]), a);
-            lap[v][f] = a;
-            ctr[v][f] = q[f][j][v][kH + p];
+            lap[j][v][f] = a;
           }
         }
+      }
+#pragma unroll
+      for (int j = 0; j < RY; j++) {
+        if (!in[j]) continue;
+        float ctr[VX][NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+          for (int v = 0; v < VX; v++) ctr[v][f] = q[f][j][v][kH + p];
 #pragma unroll
         for (int v = 0; v < VX; v++) {
           CellArgs c;
@@ -641,7 +659,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
           float pv[NP], o[NO];
 #pragma unroll
           for (int k = 0; k < NP; k++) pv[k] = pre[p][j][k][v];
-          op.cell(blk, pl, c, lap[v], ctr[v], pv, o, tmax);
+          op.cell(blk, pl, c, lap[j][v], ctr[v], pv, o, tmax);
 #pragma unroll
           for (int k = 0; k < NO; k++) pend[p][j][k][v] = o[k];
         }
