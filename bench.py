@@ -51,8 +51,9 @@ METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs F
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of this script
 # (profiles/born_fwd_adj_304cube_hbm_counters.md); reported as roofline.traffic
 # only when the measured workload and kernel are the profiled ones.
-PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 1, 8, 1, 1, 64", "f32"): 914.2e6,
-                    ("c2", "FwdOp<0, float>, 2, 8, 1, 1, 64", "f32"): 939.8e6}
+PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 2, 8, 1, 1, 64", "f32"): 1010.3e6,
+                    ("c2", "FwdOp<0, float>, 4, 4, 1, 1, 64", "f32"): 939.6e6,
+                    ("c2", "FwdOp<0, float>, 1, 8, 1, 1, 64", "f32"): 967.5e6}
 
 
 def load(name):
