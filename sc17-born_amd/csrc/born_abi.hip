@@ -25,6 +25,7 @@
 
 #include "born_hip.h"
 #include "born_launch.h"
+#include "host_par.h"
 
 using born::Geometry;
 
@@ -775,9 +776,19 @@ int born_transfer_vel_func1(born_ctx *ctx, int nx, int ny, int nz, const float *
 
 int born_transfer_vel_func2(born_ctx *ctx, int nx, int ny, int nz, const float *v) {
   if (ctx && v && nx == ctx->nx && ny == ctx->ny && nz == ctx->nz) {
-    float m = 0.f;
-    for (long long i = 0; i < ctx->n; i++) m = std::max(m, fabsf(v[i]));
-    ctx->vmax2 = m;
+    // max |v| over thread blocks: the maximum does not depend on the order
+    std::vector<float> part(born_par::threads(), 0.f);
+    const long nb = (long)part.size();
+    const long long n = ctx->n;
+    float *pp = part.data();
+    born_par::blocks(nb, 1, [=](long lo, long hi) {
+      for (long b = lo; b < hi; b++) {
+        float m = 0.f;
+        for (long long i = n * b / nb; i < n * (b + 1) / nb; i++) m = std::max(m, fabsf(v[i]));
+        pp[b] = m;
+      }
+    });
+    ctx->vmax2 = *std::max_element(part.begin(), part.end());
   }
   return vel_func(ctx, nx, ny, nz, v, ctx ? &ctx->d_vel2 : nullptr);
 }

@@ -2,6 +2,7 @@
 #include "born_op.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -9,8 +10,20 @@
 
 #include "born_hip.h"
 #include "born_host.h"
+#include "host_par.h"
 
 namespace born_host {
+
+// One progress line per shot on stderr when BORN_VERBOSE is set: a
+// full-size pass takes seconds per shot, minutes per CG iteration.
+static void shot_done(const char *pass, int ishot, int nshots) {
+  static const bool on = getenv("BORN_VERBOSE") != nullptr;
+  if (!on) return;
+  static const auto t0 = std::chrono::steady_clock::now();
+  const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fprintf(stderr, "[born] %s shot %d/%d done, %.1f s since the first line\n", pass, ishot + 1,
+          nshots, s);
+}
 
 // ------------------------------------------------------------ hypercube_float
 
@@ -44,28 +57,62 @@ static hypercube_float &as_hf(const std::shared_ptr<my_vector> &v) {
   return *h;
 }
 
-void hypercube_float::set_val(double v) { std::fill(vals.begin(), vals.end(), (float)v); }
+// Element-wise vector arithmetic runs over threads in contiguous blocks:
+// each element gets the serial loop's operations.
+template <class F>
+static void each(size_t n, F &&f) {
+  born_par::blocks((long)n, 1 << 16, [&](long lo, long hi) {
+    for (long i = lo; i < hi; i++) f((size_t)i);
+  });
+}
+
+static void fill0(std::vector<float> &v) {
+  float *p = v.data();
+  each(v.size(), [=](size_t i) { p[i] = 0.f; });
+}
+
+void hypercube_float::set_val(double v) {
+  float *p = vals.data();
+  const float f = (float)v;
+  each(vals.size(), [=](size_t i) { p[i] = f; });
+}
 
 void hypercube_float::scale(double s) {
-  for (float &x : vals) x = (float)(x * s);
+  float *p = vals.data();
+  each(vals.size(), [=](size_t i) { p[i] = (float)(p[i] * s); });
 }
 
 // hypercube_float.cc:213-222
 void hypercube_float::scale_add(double sc_me, std::shared_ptr<my_vector> other, double sc_o) {
-  const auto &o = as_hf(other).vals;
-  for (size_t i = 0; i < vals.size(); i++) vals[i] = (float)(vals[i] * sc_me + sc_o * o[i]);
+  float *p = vals.data();
+  const float *o = as_hf(other).vals.data();
+  each(vals.size(), [=](size_t i) { p[i] = (float)(p[i] * sc_me + sc_o * o[i]); });
 }
 
 void hypercube_float::add(std::shared_ptr<my_vector> other) {
-  const auto &o = as_hf(other).vals;
-  for (size_t i = 0; i < vals.size(); i++) vals[i] += o[i];
+  float *p = vals.data();
+  const float *o = as_hf(other).vals.data();
+  each(vals.size(), [=](size_t i) { p[i] += o[i]; });
 }
 
-// hypercube_float.cc:120-130: FP64 accumulation
+// hypercube_float.cc:120-130: FP64 accumulation.  Partial sums over fixed
+// chunks of 2^18 elements, added in chunk order: the result does not depend
+// on the number of threads.
 double hypercube_float::dot(std::shared_ptr<my_vector> other) {
-  const auto &o = as_hf(other).vals;
+  const float *a = vals.data(), *b = as_hf(other).vals.data();
+  const size_t n = vals.size(), chunk = size_t(1) << 18;
+  std::vector<double> part((n + chunk - 1) / chunk, 0.);
+  double *pp = part.data();
+  born_par::blocks((long)part.size(), 1, [=](long lo, long hi) {
+    for (long c = lo; c < hi; c++) {
+      double r = 0.;
+      const size_t e = std::min(n, (size_t)(c + 1) * chunk);
+      for (size_t i = (size_t)c * chunk; i < e; i++) r += (double)a[i] * (double)b[i];
+      pp[c] = r;
+    }
+  });
   double r = 0.;
-  for (size_t i = 0; i < vals.size(); i++) r += (double)vals[i] * (double)o[i];
+  for (double x : part) r += x;
   return r;
 }
 
@@ -348,9 +395,21 @@ std::shared_ptr<hypercube_float> rtm_zero_op::data_space() const {
 
 // Per-shot inputs (rtm_zero_op_3d.cc:139-174): source stamp, velocity
 // subspaces with and without the random band, receiver cells.
-rtm_zero_op::Shot rtm_zero_op::prepare_shot(int ishot) {
-  Shot sh;
+rtm_zero_op::Shot &rtm_zero_op::prepare_shot(int ishot) {
   const long long n = (long long)dom_[0].n * dom_[1].n * dom_[2].n;
+  const size_t nshots = s_.src_x.size();
+  if (shots_.size() != nshots) shots_.resize(nshots);
+  if (shots_[ishot]) return *shots_[ishot];
+  const auto &va = s_.vel_axes;
+  auto band = [&](int seed, int nr, float *out) {
+    born_host_rand_subspace(seed, nr, nr, nr, dt_, va[0].n, va[1].n, va[2].n, va[0].o, va[1].o,
+                            va[2].o, va[0].d, va[1].d, va[2].d, s_.vel.data(), dom_[0].n,
+                            dom_[1].n, dom_[2].n, dom_[0].o, dom_[1].o, dom_[2].o, dom_[0].d,
+                            dom_[1].d, dom_[2].d, out);
+  };
+  const bool keep = (double)n * sizeof(float) * nshots <= kShotCacheBytes;
+  auto fresh = std::make_unique<Shot>();
+  Shot &sh = keep ? *fresh : uncached_;
   sh.src_locs.resize(9);
   sh.src_vals.resize((size_t)9 * nt_big_);
   born_host_source_func(dom_[0].n, dom_[1].n, dom_[0].o, dom_[1].o, dom_[2].o, dom_[0].d,
@@ -358,22 +417,20 @@ rtm_zero_op::Shot rtm_zero_op::prepare_shot(int ishot) {
                         wavelet_deriv_.data(), (int)wavelet_deriv_.size(), nt_big_,
                         sh.src_locs.data(), sh.src_vals.data());
   const int seed = sh.src_locs[3] - ((int)sh.src_locs[3] / 10000) * 10000;
-  const auto &va = s_.vel_axes;
   sh.vrand.resize((size_t)n);
-  sh.vnone.resize((size_t)n);
-  for (int pass = 0; pass < 2; pass++) {
-    const int nr = pass == 0 ? s_.nbound : 0;
-    born_host_rand_subspace(seed, nr, nr, nr, dt_, va[0].n, va[1].n, va[2].n, va[0].o,
-                            va[1].o, va[2].o, va[0].d, va[1].d, va[2].d, s_.vel.data(),
-                            dom_[0].n, dom_[1].n, dom_[2].n, dom_[0].o, dom_[1].o, dom_[2].o,
-                            dom_[0].d, dom_[1].d, dom_[2].d,
-                            pass == 0 ? sh.vrand.data() : sh.vnone.data());
+  band(seed, s_.nbound, sh.vrand.data());
+  // with no band the draw never runs: the seed does not change the result
+  if (vnone_.empty()) {
+    vnone_.resize((size_t)n);
+    band(seed, 0, vnone_.data());
   }
   sh.rec_locs.resize((size_t)s_.rec_x.n * s_.rec_y.n);
   born_host_receiver_locs(s_.rec_x.n, s_.rec_y.n, s_.rec_x.o, s_.rec_y.o, s_.rec_x.d,
                           s_.rec_y.d, s_.rec_depth, dom_[0].n, dom_[1].n, dom_[0].o, dom_[1].o,
                           dom_[2].o, dom_[0].d, dom_[1].d, dom_[2].d, sh.rec_locs.data());
-  return sh;
+  if (!keep) return uncached_;
+  shots_[ishot] = std::move(fresh);
+  return *shots_[ishot];
 }
 
 // rtm_zero_op_3d.cc:101-197, one shot at a time, reading the model and
@@ -392,17 +449,17 @@ bool rtm_zero_op::forward(bool add, std::shared_ptr<my_vector> mvec,
   std::vector<float> rec_func((size_t)rnx * rny * nt_big_), shot_deriv((size_t)rnx * rny * ntd),
       shot((size_t)rnx * rny * ntd);
   for (int ishot = 0; ishot < (int)s_.src_x.size(); ishot++) {
-    Shot sh = prepare_shot(ishot);
+    Shot &sh = prepare_shot(ishot);
     prop_->transferSourceFunc(9, nt_big_, sh.src_locs, sh.src_vals.data());
-    std::fill(img.begin(), img.end(), 0.f);
+    fill0(img);
     born_host_extract_sub(va[0].n, va[1].n, va[2].n, va[0].o, va[1].o, va[2].o, va[0].d,
                           va[1].d, va[2].d, m.vals.data(), n1, n2, n3, dom_[0].o, dom_[1].o,
                           dom_[2].o, img.data());
     prop_->transferVelFunc1(n1, n2, n3, sh.vrand.data());
-    prop_->transferVelFunc2(n1, n2, n3, sh.vnone.data());
-    std::fill(rec_func.begin(), rec_func.end(), 0.f);
+    prop_->transferVelFunc2(n1, n2, n3, vnone_.data());
+    fill0(rec_func);
     prop_->transferReceiverFunc(rnx, rny, nt_big_, sh.rec_locs, rec_func.data());
-    std::fill(img_lap.begin(), img_lap.end(), 0.f);
+    fill0(img_lap);
     born_host_laplac_forward(n1, n2, n3, img.data(), img_lap.data());
     if (s_.scatter == 0)
       prop_->rtmForward(n1, n2, n3, jtd_, img_lap.data(), rec_func.data(), 9, nt_, nt_big_, rnx,
@@ -410,12 +467,14 @@ bool rtm_zero_op::forward(bool add, std::shared_ptr<my_vector> mvec,
     else
       prop_->bornForwardScatter(n1, n2, n3, jtd_, img_lap.data(), rec_func.data(), nt_,
                                 s_.scatter);
-    std::fill(shot_deriv.begin(), shot_deriv.end(), 0.f);
+    fill0(shot_deriv);
     born_host_map_forward(ntd, rnx, rny, nt_big_, rec_func.data(), shot_deriv.data());
-    std::fill(shot.begin(), shot.end(), 0.f);
+    fill0(shot);
     born_host_deriv_forward(ntd, rnx, rny, s_.t_axis.d, shot_deriv.data(), shot.data());
     float *dst = d.vals.data() + (size_t)ishot * rnx * rny * ntd;
-    for (size_t i = 0; i < shot.size(); i++) dst[i] += shot[i];
+    const float *src = shot.data();
+    each(shot.size(), [=](size_t i) { dst[i] += src[i]; });
+    shot_done("forward", ishot, (int)s_.src_x.size());
   }
   return true;
 }
@@ -435,30 +494,31 @@ bool rtm_zero_op::adjoint(bool add, std::shared_ptr<my_vector> mvec,
   std::vector<float> img((size_t)n), img_lap((size_t)n), p0((size_t)n), p1((size_t)n);
   std::vector<float> rec_func((size_t)rnx * rny * nt_big_), shot_deriv((size_t)rnx * rny * ntd);
   for (int ishot = 0; ishot < (int)s_.src_x.size(); ishot++) {
-    Shot sh = prepare_shot(ishot);
+    Shot &sh = prepare_shot(ishot);
     prop_->transferSourceFunc(9, nt_big_, sh.src_locs, sh.src_vals.data());
     prop_->transferVelFunc1(n1, n2, n3, sh.vrand.data());
-    prop_->transferVelFunc2(n1, n2, n3, sh.vnone.data());
+    prop_->transferVelFunc2(n1, n2, n3, vnone_.data());
     const float *shot = d.vals.data() + (size_t)ishot * rnx * rny * ntd;
-    std::fill(shot_deriv.begin(), shot_deriv.end(), 0.f);
+    fill0(shot_deriv);
     born_host_deriv_adjoint(ntd, rnx, rny, s_.t_axis.d, shot_deriv.data(), shot);
-    std::fill(rec_func.begin(), rec_func.end(), 0.f);
+    fill0(rec_func);
     born_host_map_adjoint(ntd, rnx, rny, nt_big_, rec_func.data(), shot_deriv.data());
     prop_->transferReceiverFunc(rnx, rny, nt_big_, sh.rec_locs, rec_func.data());
-    std::fill(img.begin(), img.end(), 0.f);
+    fill0(img);
     if (s_.mode == BornSetup::REFERENCE) {
-      std::fill(p0.begin(), p0.end(), 0.f);
-      std::fill(p1.begin(), p1.end(), 0.f);
+      fill0(p0);
+      fill0(p1);
       prop_->sourceProp(n1, n2, n3, false, true, p0.data(), p1.data(), jts_, 9, nt_);
       prop_->rtmAdjoint(n1, n2, n3, jtd_, p1.data(), p0.data(), img.data(), 9, nt_);
     } else {
       prop_->bornAdjointExact(n1, n2, n3, jtd_, rec_func.data(), img.data(), nt_, s_.scatter);
     }
-    std::fill(img_lap.begin(), img_lap.end(), 0.f);
+    fill0(img_lap);
     born_host_laplac_adjoint(n1, n2, n3, img_lap.data(), img.data());
     born_host_add_image(va[0].n, va[1].n, va[2].n, va[0].o, va[1].o, va[2].o, va[0].d, va[1].d,
                         va[2].d, m.vals.data(), n1, n2, n3, dom_[0].o, dom_[1].o, dom_[2].o,
                         img_lap.data());
+    shot_done("adjoint", ishot, (int)s_.src_x.size());
   }
   return true;
 }

@@ -229,9 +229,17 @@ class rtm_zero_op : public my_operator {
  private:
   struct Shot {
     std::vector<int> src_locs, rec_locs;
-    std::vector<float> src_vals, vrand, vnone;
+    std::vector<float> src_vals, vrand;
   };
-  Shot prepare_shot(int ishot);
+  // Source stamp, receiver cells and random-boundary velocity of one shot.
+  // Velocity and geometry are fixed for the operator's life, so shots are
+  // prepared once and kept while all shots' velocities fit in
+  // kShotCacheBytes of host memory (else prepared again on each call).
+  Shot &prepare_shot(int ishot);
+  static constexpr double kShotCacheBytes = 64.0 * (1 << 30);
+  std::vector<std::unique_ptr<Shot>> shots_;
+  Shot uncached_;
+  std::vector<float> vnone_;  // velocity without the random band: one for all shots
   std::shared_ptr<baseProp> prop_;
   BornSetup s_;
   axis dom_[3];

@@ -6,6 +6,8 @@
 #include <cstdlib>
 #include <vector>
 
+#include "host_par.h"
+
 namespace {
 
 // Levinson solution of the symmetric Toeplitz system r f = g, with the
@@ -178,63 +180,84 @@ void born_host_receiver_locs(int rec_nx, int rec_ny, float ro1, float ro2, float
     }
 }
 
+// Host loops below run over threads in blocks of whole traces, planes or
+// elements; each output element sees the serial loop's operations in the
+// same order.
 void born_host_map_forward(int nt, int n2, int n3, int ntb, const float *m, float *d) {
-  for (long tr = 0; tr < (long)n2 * n3; tr++)
-    for (int it = 0; it < nt; it++) d[tr * nt + it] += m[tr * ntb + it + 4];
+  born_par::blocks((long)n2 * n3, 64, [=](long lo, long hi) {
+    for (long tr = lo; tr < hi; tr++)
+      for (int it = 0; it < nt; it++) d[tr * nt + it] += m[tr * ntb + it + 4];
+  });
 }
 
 void born_host_map_adjoint(int nt, int n2, int n3, int ntb, float *m, const float *d) {
-  for (long tr = 0; tr < (long)n2 * n3; tr++)
-    for (int it = 0; it < nt; it++) m[tr * ntb + it + 4] += d[tr * nt + it];
+  born_par::blocks((long)n2 * n3, 64, [=](long lo, long hi) {
+    for (long tr = lo; tr < hi; tr++)
+      for (int it = 0; it < nt; it++) m[tr * ntb + it + 4] += d[tr * nt + it];
+  });
 }
 
 // The reference's time derivative adds the first-sample difference three
 // times to trace 0 of each plane and twice to every other trace
 // (deriv_3d.cc:40-55); kept as is since it is self-adjoint-consistent.
 void born_host_deriv_forward(int n1, int n2, int n3, float d1, const float *m, float *d) {
-  for (int i3 = 0; i3 < n3; i3++) {
-    float *dp = d + (long)i3 * n1 * n2;
-    const float *mp = m + (long)i3 * n1 * n2;
-    dp[1] += mp[0] / d1;
-    dp[0] -= mp[0] / d1;
-    for (int i2 = 0; i2 < n2; i2++) {
-      float *dt = dp + (long)i2 * n1;
-      const float *mt = mp + (long)i2 * n1;
-      dt[1] += mt[0] / d1;
-      dt[0] -= mt[0] / d1;
-      for (int i1 = 0; i1 + 1 < n1; i1++) {
-        dt[i1] -= mt[i1] / d1;
-        dt[i1 + 1] += mt[i1] / d1;
+  born_par::blocks(n3, 1, [=](long p0, long p1) {
+    for (long i3 = p0; i3 < p1; i3++) {
+      float *dp = d + i3 * n1 * n2;
+      const float *mp = m + i3 * n1 * n2;
+      dp[1] += mp[0] / d1;
+      dp[0] -= mp[0] / d1;
+      for (int i2 = 0; i2 < n2; i2++) {
+        float *dt = dp + (long)i2 * n1;
+        const float *mt = mp + (long)i2 * n1;
+        dt[1] += mt[0] / d1;
+        dt[0] -= mt[0] / d1;
+        for (int i1 = 0; i1 + 1 < n1; i1++) {
+          dt[i1] -= mt[i1] / d1;
+          dt[i1 + 1] += mt[i1] / d1;
+        }
       }
     }
-  }
+  });
 }
 
 void born_host_deriv_adjoint(int n1, int n2, int n3, float d1, float *m, const float *d) {
-  for (int i3 = 0; i3 < n3; i3++) {
-    float *mp = m + (long)i3 * n1 * n2;
-    const float *dp = d + (long)i3 * n1 * n2;
-    mp[0] += (dp[1] - dp[0]) / d1;
-    for (int i2 = 0; i2 < n2; i2++) {
-      float *mt = mp + (long)i2 * n1;
-      const float *dt = dp + (long)i2 * n1;
-      mt[0] += (dt[1] - dt[0]) / d1;
-      for (int i1 = 0; i1 + 1 < n1; i1++) mt[i1] += (dt[i1 + 1] - dt[i1]) / d1;
+  born_par::blocks(n3, 1, [=](long p0, long p1) {
+    for (long i3 = p0; i3 < p1; i3++) {
+      float *mp = m + i3 * n1 * n2;
+      const float *dp = d + i3 * n1 * n2;
+      mp[0] += (dp[1] - dp[0]) / d1;
+      for (int i2 = 0; i2 < n2; i2++) {
+        float *mt = mp + (long)i2 * n1;
+        const float *dt = dp + (long)i2 * n1;
+        mt[0] += (dt[1] - dt[0]) / d1;
+        for (int i1 = 0; i1 + 1 < n1; i1++) mt[i1] += (dt[i1 + 1] - dt[i1]) / d1;
+      }
     }
-  }
+  });
 }
 
+// The serial form is d[i + plane] += m[i]; d[i] -= m[i] for i below
+// plane (n3 - 1): element j gets + m[j - plane] first, then - m[j].
 void born_host_laplac_forward(int n1, int n2, int n3, const float *m, float *d) {
-  const long plane = (long)n1 * n2;
-  for (long i = 0; i < plane * (n3 - 1); i++) {
-    d[i + plane] += m[i];
-    d[i] -= m[i];
-  }
+  const long plane = (long)n1 * n2, last = plane * (n3 - 1);
+  if (n3 < 2) return;
+  born_par::blocks(plane * n3, 1 << 16, [=](long lo, long hi) {
+    for (long j = lo; j < hi; j++) {
+      float v = d[j];
+      if (j >= plane) v += m[j - plane];
+      if (j < last) v -= m[j];
+      d[j] = v;
+    }
+  });
 }
 
 void born_host_laplac_adjoint(int n1, int n2, int n3, float *m, const float *d) {
   const long plane = (long)n1 * n2;
-  for (long i = 0; i < plane * (n3 - 1); i++) m[i] += d[i + plane] - d[i];
+  if (n3 < 2) return;
+  born_par::blocks(plane * (n3 - 1), 1 << 16, [=](long lo, long hi) {
+    for (long i = lo; i < hi; i++) m[i] += d[i + plane] - d[i];
+  });
 }
 
 static void window(int n1o, int n2o, int n3o, float o1o, float o2o, float o3o,
@@ -243,16 +266,21 @@ static void window(int n1o, int n2o, int n3o, float o1o, float o2o, float o3o,
                    bool accumulate_into_image) {
   const int f1 = (int)((o1i - o1o) / d1o), f2 = (int)((o2i - o2o) / d2o),
             f3 = (int)((o3i - o3o) / d3o);
-  for (int i3 = std::max(0, -f3); i3 < std::min(n3i, n3o - f3); i3++)
-    for (int i2 = std::max(0, -f2); i2 < std::min(n2i, n2o - f2); i2++)
-      for (int i1 = std::max(0, -f1); i1 < std::min(n1i, n1o - f1); i1++) {
-        float &o = image[i1 + f1 + (long)(i2 + f2) * n1o + (long)(i3 + f3) * n1o * n2o];
-        float &in = img[i1 + (long)i2 * n1i + (long)i3 * n1i * n2i];
-        if (accumulate_into_image)
-          o += in;
-        else
-          in = o;
-      }
+  const int z0 = std::max(0, -f3), z1 = std::min(n3i, n3o - f3);
+  const int y0 = std::max(0, -f2), y1 = std::min(n2i, n2o - f2);
+  const int x0 = std::max(0, -f1), x1 = std::min(n1i, n1o - f1);
+  born_par::blocks(z1 - z0, 1, [=](long lo, long hi) {
+    for (long i3 = z0 + lo; i3 < z0 + hi; i3++)
+      for (int i2 = y0; i2 < y1; i2++)
+        for (int i1 = x0; i1 < x1; i1++) {
+          float &o = image[i1 + f1 + (long)(i2 + f2) * n1o + (i3 + f3) * n1o * n2o];
+          float &in = img[i1 + (long)i2 * n1i + i3 * n1i * n2i];
+          if (accumulate_into_image)
+            o += in;
+          else
+            in = o;
+        }
+  });
 }
 
 void born_host_extract_sub(int n1o, int n2o, int n3o, float o1o, float o2o, float o3o,

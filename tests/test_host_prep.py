@@ -65,7 +65,10 @@ def test_rand_subspace_bit_exact(H, nrand):
     assert np.array_equal(a, b)
 
 
-def test_receivers_and_linear_trace_ops_bit_exact(H):
+# the large case splits every loop over host threads (traces, planes and
+# 2^16-element blocks); outputs start non-zero since every op accumulates
+@pytest.mark.parametrize("big", [False, True])
+def test_receivers_and_linear_trace_ops_bit_exact(H, big):
     rng = np.random.default_rng(2)
     la, lb = np.zeros(35, np.int32), np.zeros(35, np.int32)
     H.born_host_receiver_locs(7, 5, 10.0, 15.0, 20.0, 20.0, 10.0, 40, 36, -240.0, -240.0,
@@ -73,20 +76,20 @@ def test_receivers_and_linear_trace_ops_bit_exact(H):
     L().orc_receiver_locs(7, 5, 10.0, 15.0, 20.0, 20.0, 10.0, 40, 36, -240.0, -240.0, -240.0,
                           10.0, 10.0, 10.0, lb)
     assert np.array_equal(la, lb)
-    nt, n2, n3, ntb = 20, 4, 3, 28
+    nt, n2, n3, ntb = (50, 40, 30, 58) if big else (20, 4, 3, 28)
+    lap = (40, 30, 80) if big else (5, 4, 6)
+    nl = int(np.prod(lap))
     m = rng.standard_normal(ntb * n2 * n3).astype(np.float32)
     d = rng.standard_normal(nt * n2 * n3).astype(np.float32)
+    z = (lambda a: rng.standard_normal(a.size).astype(np.float32)) if big else np.zeros_like
+    q = rng.standard_normal(nl).astype(np.float32)
     pairs = [
-        ("born_host_map_forward", "orc_map_data_forward", (nt, n2, n3, ntb), m, np.zeros_like(d)),
-        ("born_host_map_adjoint", "orc_map_data_adjoint", (nt, n2, n3, ntb), np.zeros_like(m), d),
-        ("born_host_deriv_forward", "orc_deriv_forward", (nt, n2, n3, 0.004), d,
-         np.zeros_like(d)),
-        ("born_host_deriv_adjoint", "orc_deriv_adjoint", (nt, n2, n3, 0.004), np.zeros_like(d),
-         d),
-        ("born_host_laplac_forward", "orc_laplac_forward", (5, 4, 6), d[:120],
-         np.zeros(120, np.float32)),
-        ("born_host_laplac_adjoint", "orc_laplac_adjoint", (5, 4, 6), np.zeros(120, np.float32),
-         d[:120]),
+        ("born_host_map_forward", "orc_map_data_forward", (nt, n2, n3, ntb), m, z(d)),
+        ("born_host_map_adjoint", "orc_map_data_adjoint", (nt, n2, n3, ntb), z(m), d),
+        ("born_host_deriv_forward", "orc_deriv_forward", (nt, n2, n3, 0.004), d, z(d)),
+        ("born_host_deriv_adjoint", "orc_deriv_adjoint", (nt, n2, n3, 0.004), z(d), d),
+        ("born_host_laplac_forward", "orc_laplac_forward", lap, q, z(q)),
+        ("born_host_laplac_adjoint", "orc_laplac_adjoint", lap, z(q), q),
     ]
     for hname, oname, dims, x, y in pairs:
         fa = getattr(H, hname)
