@@ -219,8 +219,8 @@ def main():
     ap.add_argument("--batch-nt", type=int, default=0, help="time steps of the batch leg")
     ap.add_argument("--batch-shots", type=int, default=0, help="shots of the batch leg")
     ap.add_argument("--fwd-shape", default="",
-                    help="fix the FP32 forward launch shape: rows,waves,cells_per_lane "
-                         "(64-cell rows; profiling runs), e.g. 1,8,1")
+                    help="fix the FP32 forward launch shape: rows,waves,cells_per_lane"
+                         "[,lookahead] (64-cell rows; profiling runs), e.g. 1,8,1 or 2,8,1,2")
     ap.add_argument("--step-pairs", type=int, default=0,
                     help="FP32 forward in two-step launches: 0 off, 1 when timed faster, 2 on")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -267,8 +267,9 @@ def main():
         P.set_zchunk(args.zchunk)
     P.set_step_pairs(args.step_pairs)
     if args.fwd_shape:
-        ry, wy, vx = (int(v) for v in args.fwd_shape.split(","))
-        P.set_tuning("fwd", ry, wy, vx, 1, args.zchunk or 64)
+        sh = [int(v) for v in args.fwd_shape.split(",")]
+        ry, wy, vx, la = sh + [1] * (4 - len(sh))
+        P.set_tuning("fwd", ry, wy, vx, la, args.zchunk or 64)
     if args.budget_gb:
         P.set_checkpoint_budget(int(args.budget_gb * 2 ** 30))
     model = torch.from_numpy(p.model).to(dev)
@@ -336,7 +337,7 @@ def main():
     alg_gbs = born_bytes("f32", nt, recomputed) * cell_steps / elapsed / 1e9
     ft = tuning["fwd"]
     fwd_kernel = "FwdOp<0, float>, %d, %d, %d, %d, %d" % (
-        ft["rows_per_thread"], ft["waves"], ft["cells_per_lane"], ft["planes_per_step"],
+        ft["rows_per_thread"], ft["waves"], ft["cells_per_lane"], ft["lookahead"],
         64 // ft["cells_per_lane"])
     traffic = PROFILED_TRAFFIC.get((args.config, fwd_kernel, "f32"))
     if args.nt or args.shots:

@@ -187,15 +187,16 @@ int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
 
 /* The chosen tile of a step kernel: tile_x x tile_y cells of the xy plane,
  * cells_per_lane consecutive x cells per thread (2: one 8-byte / 4-byte load
- * of a pair; needs an even nx), planes_per_step z planes advanced between two
- * waits for memory. */
+ * of a pair; needs an even nx), lookahead = how many z planes ahead of the
+ * plane being computed the loads are issued (1 or 2; 1 for the two-step
+ * kernels). */
 int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y,
-                  int *cells_per_lane, int *planes_per_step);
+                  int *cells_per_lane, int *lookahead);
 /* Fix the launch shape of a step kernel (rows per thread x waves per block
- * x cells per lane x planes per step, one of the autotuner's shapes) and its
- * z chunk; the autotuner then leaves that kernel. */
+ * x cells per lane x lookahead, one of the autotuner's shapes) and its z
+ * chunk; the autotuner then leaves that kernel. */
 int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_per_block,
-                    int cells_per_lane, int planes_per_step, int zchunk);
+                    int cells_per_lane, int lookahead, int zchunk);
 
 void *born_get_stream(const born_ctx *ctx);
 

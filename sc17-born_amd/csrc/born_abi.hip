@@ -1473,33 +1473,33 @@ int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
 }
 
 int born_get_tile(const born_ctx *ctx, int kernel, int *tile_x, int *tile_y,
-                  int *cells_per_lane, int *planes_per_step) {
+                  int *cells_per_lane, int *lookahead) {
   if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
   if (is_pair(kernel)) {
     const born::PairShape &ps = born::kPairShapes[tuning(ctx, kernel).shape];
     if (tile_x) *tile_x = born::kTX - 2 * born::kH;
     if (tile_y) *tile_y = ps.ry * ps.wy;
     if (cells_per_lane) *cells_per_lane = 1;
-    if (planes_per_step) *planes_per_step = 1;
+    if (lookahead) *lookahead = 1;
     return BORN_OK;
   }
   const born::TileShape &ts = born::kShapes[tuning(ctx, kernel).shape];
   if (tile_x) *tile_x = ts.lx * ts.vx;
   if (tile_y) *tile_y = ts.ry * (born::kTX * ts.wy / ts.lx);
   if (cells_per_lane) *cells_per_lane = ts.vx;
-  if (planes_per_step) *planes_per_step = ts.pz;
+  if (lookahead) *lookahead = ts.la;
   return BORN_OK;
 }
 
 int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_per_block,
-                    int cells_per_lane, int planes_per_step, int zchunk) {
+                    int cells_per_lane, int lookahead, int zchunk) {
   if (!ctx || kernel < 0 || kernel >= K_NUM) return BORN_ERR_ARG;
   if (zchunk < 1) return fail(ctx, BORN_ERR_ARG, "zchunk must be positive");
   if (is_pair(kernel)) {
     for (int sh = 0; sh < born::kNumPairShapes; sh++) {
       const born::PairShape &ps = born::kPairShapes[sh];
       if (ps.ry == rows_per_thread && ps.wy == waves_per_block && cells_per_lane == 1 &&
-          planes_per_step == 1) {
+          lookahead == 1) {
         ctx->tune[kernel] = {sh, zchunk};
         ctx->tune_ms[kernel] = 0.f;
         return BORN_OK;
@@ -1510,7 +1510,7 @@ int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_pe
   for (int sh = 0; sh < born::kNumShapes; sh++) {
     const born::TileShape &ts = born::kShapes[sh];
     if (ts.ry == rows_per_thread && ts.wy == waves_per_block && ts.vx == cells_per_lane &&
-        ts.pz == planes_per_step) {
+        ts.la == lookahead) {
       ctx->tune[kernel] = {sh, zchunk};
       ctx->tune_ms[kernel] = 0.f;  // fixed: the autotuner leaves it
       return BORN_OK;

@@ -72,6 +72,41 @@ __device__ __forceinline__ const T &at(const T *p, uint32_t o) {
   return *(const T *)((const char *)p + (uint64_t)(o * (uint32_t)sizeof(T)));
 }
 
+// ---- Stores the compiler's wait-counter model does not see.  Loads and
+// stores share one counter (vmcnt) on this architecture and the compiler
+// takes a counter with both pending as unordered, so any wait for a load
+// would drain every load in flight.  As inline asm the step kernels' stores
+// are invisible to it: a wait for a load issued two steps earlier can leave
+// the previous step's loads in flight.  That wait stays exact because a
+// wavefront's loads return in order, so a vmcnt that counts only the loads
+// issued after the one waited for covers it (pending stores only make the
+// wait longer).  No step kernel reads what it stores before the launch ends,
+// and a store of at most 8 bytes reads its data registers at issue.  The
+// base is a wave-uniform pointer (scalar registers), off a byte offset.
+// A vector memory instruction that reads a scalar register written by a
+// vector instruction (v_readlane, as in a reload of a spilled scalar) needs
+// 5 wait states before it, which the compiler does not insert for inline
+// asm: each store starts with s_nop 4.
+__device__ __forceinline__ void st_b16(void *p, uint32_t off, unsigned short v) {
+  asm volatile("s_nop 4\n\tglobal_store_short %0, %1, %2"
+               : : "v"(off), "v"((uint32_t)v), "s"(p) : "memory");
+}
+__device__ __forceinline__ void st_b32(void *p, uint32_t off, uint32_t v) {
+  asm volatile("s_nop 4\n\tglobal_store_dword %0, %1, %2"
+               : : "v"(off), "v"(v), "s"(p) : "memory");
+}
+__device__ __forceinline__ void st_b64(void *p, uint32_t off, uint64_t v) {
+  asm volatile("s_nop 4\n\tglobal_store_dwordx2 %0, %1, %2"
+               : : "v"(off), "v"(v), "s"(p) : "memory");
+}
+
+// Block barrier for LDS: the block's LDS accesses complete, then s_barrier.
+// __syncthreads also fences global memory, which waits for every load in
+// flight; the step kernels share nothing through global memory in a launch.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" : : : "memory");
+}
+
 // ---- storage of wavefield levels: float, or FP16 with a power-of-two scale
 // per stored level (value = half * inv_scale).  Conversions saturate at the
 // FP16 range instead of overflowing.
@@ -260,15 +295,16 @@ __device__ __forceinline__ void ld_pair(const __half2 *p, uint32_t o, const floa
 template <int VX>
 __device__ __forceinline__ void st_pair(__half2 *p, uint32_t o, const float (&a)[VX],
                                         const float (&b)[VX], const float (&scale)[2]) {
+  const __half2 c0 = __floats2half2_rn(sat_half(a[0] * scale[0]), sat_half(b[0] * scale[1]));
+  uint32_t u0;
+  __builtin_memcpy(&u0, &c0, 4);
   if constexpr (VX == 1) {
-    at(p, o) = __floats2half2_rn(sat_half(a[0] * scale[0]), sat_half(b[0] * scale[1]));
+    st_b32(p, o * 4u, u0);
   } else {
-    const __half2 c0 = __floats2half2_rn(sat_half(a[0] * scale[0]), sat_half(b[0] * scale[1]));
     const __half2 c1 = __floats2half2_rn(sat_half(a[1] * scale[0]), sat_half(b[1] * scale[1]));
-    uint2 u;
-    __builtin_memcpy(&u.x, &c0, 4);
-    __builtin_memcpy(&u.y, &c1, 4);
-    at((uint2 *)p, o >> 1) = u;
+    uint32_t u1;
+    __builtin_memcpy(&u1, &c1, 4);
+    st_b64(p, o * 4u, (uint64_t)u1 << 32 | u0);
   }
 }
 
@@ -299,18 +335,21 @@ __device__ __forceinline__ void ld_vec(const __half *p, uint32_t o, float inv,
 template <int VX>
 __device__ __forceinline__ void st_vec(float *p, uint32_t o, const float (&v)[VX], float) {
   if constexpr (VX == 1)
-    at(p, o) = v[0];
+    st_b32(p, o * 4u, __float_as_uint(v[0]));
   else
-    at((float2 *)p, o >> 1) = make_float2(v[0], v[1]);
+    st_b64(p, o * 4u, (uint64_t)__float_as_uint(v[1]) << 32 | __float_as_uint(v[0]));
 }
 template <int VX>
 __device__ __forceinline__ void st_vec(__half *p, uint32_t o, const float (&v)[VX],
                                        float scale) {
-  if constexpr (VX == 1)
-    at(p, o) = __float2half_rn(sat_half(v[0] * scale));
-  else
-    at((__half2 *)p, o >> 1) =
-        __floats2half2_rn(sat_half(v[0] * scale), sat_half(v[1] * scale));
+  if constexpr (VX == 1) {
+    st_b16(p, o * 2u, __half_as_ushort(__float2half_rn(sat_half(v[0] * scale))));
+  } else {
+    const __half2 c = __floats2half2_rn(sat_half(v[0] * scale), sat_half(v[1] * scale));
+    uint32_t u;
+    __builtin_memcpy(&u, &c, 4);
+    st_b32(p, o * 2u, u);
+  }
 }
 
 // Generic z-marching leapfrog tile.  VX cells per lane along x (a tile is
@@ -328,7 +367,12 @@ __device__ __forceinline__ void st_vec(__half *p, uint32_t o, const float (&v)[V
 //             float (&tmax)[NF]);                the new values of one cell
 //   storev<VX>(const Blk&, uint32_t o, const float (&out)[NOUT][VX])
 //   void publish_max(int f, float m);           (NMAX > 0) block max of |new|
-template <class Op, int RY, int WY, int VX, int PZ, int LX>
+template <int B>
+struct StepSet {
+  static constexpr int value = B;
+};
+
+template <class Op, int RY, int WY, int VX, int PZ, int LX, int D>
 __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const Sponge &sp,
                                            const Op &op, const int L) {
   constexpr int NF = Op::NF;
@@ -414,22 +458,23 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     hoff[k] = (uint32_t)gy * g.nx + gx;
   }
 
-  // The tile marches PZ planes per step: the loads of the next PZ planes
-  // (queue leads, LDS halos, pointwise inputs) are issued before this step's
-  // PZ planes are computed, and one wait and two barriers serve PZ planes.
-  // (Stores and loads share one counter on this architecture, so with stores
-  // pending any wait for a load drains every load issued so far; the loads
-  // in flight per wait are raised by taking more planes per step, not by a
-  // deeper register pipeline.)
+  // The tile marches PZ planes per step.  The loads of step z + D*PZ (queue
+  // leads, LDS halos, pointwise inputs) are issued before step z is computed:
+  // with D = 2 a load has two steps' compute to arrive, and the wait at a
+  // step's start leaves the previous step's loads in flight (the stores are
+  // invisible to the compiler's wait model and the barriers do not fence
+  // global memory, see st_b32 and lds_barrier).  Loads past the chunk's last
+  // step are clamped to planes the chunk reads anyway.
   //
   // The LDS halo of a plane is loaded together with the plane's centre rows
-  // (when the plane enters the queue as a lead), not kH planes later: the
+  // (when the plane enters the z queue as a lead), not kH planes later: the
   // neighbouring tiles read those rows at about the same time, so they come
   // from L2.  It waits in its LDS ring slot, not in registers; hl holds the
   // halos of the leads in flight.
+  static_assert(D == 1 || PZ == 1, "a two-step lookahead takes one plane per step");
   float q[NF][RY][VX][NQ];
-  float lead[PZ][NF][RY][VX];
-  float hl[PZ][NF][HPT][VX];
+  float lead[D][PZ][NF][RY][VX];
+  float hl[D][PZ][NF][HPT][VX];
   auto put_halo = [&](int slot, int f, int k, const float (&h)[VX]) {
     float *d = &tile[slot][f][hr[k]][hc[k]];
     if constexpr (VX == 1)
@@ -437,7 +482,8 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     else
       *(float2 *)d = make_float2(h[0], h[1]);
   };
-  float pre[PZ][RY][NP][VX], pre_n[PZ][RY][NP][VX];
+  // pointwise inputs of the steps in flight, and of the step computed
+  float pre[D][PZ][RY][NP][VX], cur[PZ][RY][NP][VX];
   float tmax[NF];
   // every field of the current level at element offset o
   auto ld_all = [&](uint32_t o, float (&v)[NF][VX]) {
@@ -446,6 +492,34 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     } else {
 #pragma unroll
       for (int f = 0; f < NF; f++) op.template ldv<VX>(blk, f, o, v[f]);
+    }
+  };
+  // the loads of the step at plane zs into buffer set b
+  auto issue = [&](int zs, int b) {
+#pragma unroll
+    for (int p = 0; p < PZ; p++) {
+      const uint32_t zl = (uint32_t)min(zs + kH + p, z1 - 1 + kH) * g.plane;
+      const uint32_t zn = (uint32_t)min(zs + p, z1 - 1) * g.plane;
+#pragma unroll
+      for (int j = 0; j < RY; j++) {
+        float w[NF][VX];
+        ld_all(zl + row[j], w);
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+          for (int v = 0; v < VX; v++) lead[b][p][f][j][v] = w[f][v];
+      }
+#pragma unroll
+      for (int k = 0; k < HPT; k++) {
+        float w[NF][VX];
+        ld_all(zl + hoff[k], w);
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+          for (int v = 0; v < VX; v++) hl[b][p][f][k][v] = w[f][v];
+      }
+#pragma unroll
+      for (int j = 0; j < RY; j++) op.template prefetchv<VX>(blk, pre[b][p][j], zn + row[j]);
     }
   };
 #pragma unroll
@@ -461,40 +535,20 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
 #pragma unroll
         for (int v = 0; v < VX; v++) q[f][j][v][k] = w[f][v];
     }
-#pragma unroll
-    for (int p = 0; p < PZ; p++) {
-      float w[NF][VX];
-      ld_all((uint32_t)min(z0 + kH + p, g.nz - 1) * g.plane + row[j], w);
-#pragma unroll
-      for (int f = 0; f < NF; f++)
-#pragma unroll
-        for (int v = 0; v < VX; v++) lead[p][f][j][v] = w[f][v];
-    }
   }
 #pragma unroll
-  for (int p = 0; p < kH + PZ; p++) {
-    const int zp = min(z0 + p, g.nz - 1);
+  for (int p = 0; p < kH; p++) {
 #pragma unroll
     for (int k = 0; k < HPT; k++) {
       float h[NF][VX];
-      ld_all((uint32_t)zp * g.plane + hoff[k], h);
+      ld_all((uint32_t)(z0 + p) * g.plane + hoff[k], h);
 #pragma unroll
-      for (int f = 0; f < NF; f++) {
-        if (p < kH) {
-          if (hv[k]) put_halo((z0 + p) % NR, f, k, h[f]);
-        } else {
-#pragma unroll
-          for (int v = 0; v < VX; v++) hl[p - kH][f][k][v] = h[f][v];
-        }
-      }
+      for (int f = 0; f < NF; f++)
+        if (hv[k]) put_halo((z0 + p) % NR, f, k, h[f]);
     }
   }
 #pragma unroll
-  for (int p = 0; p < PZ; p++)
-#pragma unroll
-    for (int j = 0; j < RY; j++)
-      op.template prefetchv<VX>(blk, pre[p][j],
-                                (uint32_t)min(z0 + p, g.nz - 1) * g.plane + row[j]);
+  for (int b = 0; b < D; b++) issue(z0 + b * PZ, b);
 
   // The new values of a step are stored at the start of the next step,
   // after its wait and before its loads: the wait for a step's inputs then
@@ -514,8 +568,19 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
     }
   };
 
-  for (int z = z0; z < z1; z += PZ) {
-    // stage the step's planes: queue leads, centre and halo into LDS
+  // one step at plane z from buffer set b (a compile-time constant)
+  auto step = [&](const int z, auto bc) {
+    constexpr int b = decltype(bc)::value;
+    // stage the step's planes: queue leads, centre and halo into LDS, and
+    // the pointwise inputs out of the buffer the next loads refill
+#pragma unroll
+    for (int p = 0; p < PZ; p++)
+#pragma unroll
+      for (int j = 0; j < RY; j++)
+#pragma unroll
+        for (int k = 0; k < NP; k++)
+#pragma unroll
+          for (int v = 0; v < VX; v++) cur[p][j][k][v] = pre[b][p][j][k][v];
 #pragma unroll
     for (int f = 0; f < NF; f++) {
 #pragma unroll
@@ -523,7 +588,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
 #pragma unroll
         for (int p = 0; p < PZ; p++)
 #pragma unroll
-          for (int v = 0; v < VX; v++) q[f][j][v][2 * kH + p] = lead[p][f][j][v];
+          for (int v = 0; v < VX; v++) q[f][j][v][2 * kH + p] = lead[b][p][f][j][v];
 #pragma unroll
         for (int p = 0; p < PZ; p++) {
           float *d = &tile[(z + p) % NR][f][kH + RY * ty + j][kH + VX * tx];
@@ -537,39 +602,11 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
       for (int p = 0; p < PZ; p++)
 #pragma unroll
         for (int k = 0; k < HPT; k++)
-          if (hv[k]) put_halo((z + kH + p) % NR, f, k, hl[p][f][k]);
+          if (hv[k]) put_halo((z + kH + p) % NR, f, k, hl[b][p][f][k]);
     }
     flush();
-    // the next step's loads, in flight during this step's compute: queue
-    // leads z+PZ+kH.. with their halos, and the pointwise inputs of z+PZ..
-    if (z + PZ < z1) {
-#pragma unroll
-      for (int p = 0; p < PZ; p++) {
-        const uint32_t zl = (uint32_t)min(z + PZ + kH + p, g.nz - 1) * g.plane;
-        const uint32_t zn = (uint32_t)min(z + PZ + p, g.nz - 1) * g.plane;
-#pragma unroll
-        for (int j = 0; j < RY; j++) {
-          float w[NF][VX];
-          ld_all(zl + row[j], w);
-#pragma unroll
-          for (int f = 0; f < NF; f++)
-#pragma unroll
-            for (int v = 0; v < VX; v++) lead[p][f][j][v] = w[f][v];
-        }
-#pragma unroll
-        for (int k = 0; k < HPT; k++) {
-          float w[NF][VX];
-          ld_all(zl + hoff[k], w);
-#pragma unroll
-          for (int f = 0; f < NF; f++)
-#pragma unroll
-            for (int v = 0; v < VX; v++) hl[p][f][k][v] = w[f][v];
-        }
-#pragma unroll
-        for (int j = 0; j < RY; j++) op.template prefetchv<VX>(blk, pre_n[p][j], zn + row[j]);
-      }
-    }
-    __syncthreads();
+    issue(z + D * PZ, b);
+    lds_barrier();
 #pragma unroll
     for (int p = 0; p < PZ; p++) {
       const int zz = z + p;
@@ -658,7 +695,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
           c.b = fminf(bxy[j][v], bzv);
           float pv[NP], o[NO];
 #pragma unroll
-          for (int k = 0; k < NP; k++) pv[k] = pre[p][j][k][v];
+          for (int k = 0; k < NP; k++) pv[k] = cur[p][j][k][v];
           op.cell(blk, pl, c, lap[j][v], ctr[v], pv, o, tmax);
 #pragma unroll
           for (int k = 0; k < NO; k++) pend[p][j][k][v] = o[k];
@@ -666,7 +703,7 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
       }
     }
     pend_z = z;
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int f = 0; f < NF; f++)
 #pragma unroll
@@ -675,14 +712,14 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
         for (int v = 0; v < VX; v++)
 #pragma unroll
           for (int k = 0; k < 2 * kH; k++) q[f][j][v][k] = q[f][j][v][k + PZ];
-#pragma unroll
-    for (int p = 0; p < PZ; p++)
-#pragma unroll
-      for (int j = 0; j < RY; j++)
-#pragma unroll
-        for (int k = 0; k < NP; k++)
-#pragma unroll
-          for (int v = 0; v < VX; v++) pre[p][j][k][v] = pre_n[p][j][k][v];
+  };
+
+  for (int z = z0; z < z1; z += D * PZ) {
+    step(z, StepSet<0>{});
+    if constexpr (D == 2) {
+      if (z + PZ >= z1) break;
+      step(z + PZ, StepSet<1>{});
+    }
   }
   flush();
   // block maxima of |new level| for the next step's FP16 scales
@@ -706,11 +743,12 @@ __device__ __forceinline__ void march_tile(const Grid &g, const Taps &t, const S
   }
 }
 
-template <class Op, int RY, int WY, int VX, int PZ, int LX>
+// The step kernels take one plane per step; LA steps of load lookahead.
+template <class Op, int RY, int WY, int VX, int LA, int LX>
 __global__ void __launch_bounds__(kTX *WY)
     march_kernel(const Grid g, const Taps t, const Sponge sp, const Op op) {
-  march_tile<Op, RY, WY, VX, PZ, LX>(g, t, sp, op,
-                                     xcd_remap((int)blockIdx.x, (int)gridDim.x));
+  march_tile<Op, RY, WY, VX, 1, LX, LA>(g, t, sp, op,
+                                        xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // Tiles of a batch of shots with their own grids: shot s owns the logical
@@ -739,14 +777,14 @@ __host__ __device__ inline Grid tile_grid(int nx, int ny, int nz, int tw, int ty
 // One launch advances every shot of a batch by one step.  BOp::shot(s, tw,
 // ty, zc) returns {Grid, Sponge, Op} of shot s with the op's pointers moved to
 // the shot's part of the batch arrays.
-template <class BOp, int RY, int WY, int VX, int PZ, int LX>
+template <class BOp, int RY, int WY, int VX, int LA, int LX>
 __global__ void __launch_bounds__(kTX *WY)
     march_batch_kernel(const BatchTiles bt, const Taps t, const BOp bop) {
   const int Lg = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   int s = 0;
   while (s + 1 < bt.n && Lg >= bt.tile0[s + 1]) s++;
   const auto sh = bop.shot(s, LX * VX, RY * (kTX * WY / LX), bt.zc);
-  march_tile<typename BOp::Op, RY, WY, VX, PZ, LX>(sh.g, t, sh.sp, sh.op, Lg - bt.tile0[s]);
+  march_tile<typename BOp::Op, RY, WY, VX, 1, LX, LA>(sh.g, t, sh.sp, sh.op, Lg - bt.tile0[s]);
 }
 
 }  // namespace born

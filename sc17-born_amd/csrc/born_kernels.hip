@@ -607,22 +607,52 @@ __global__ void transpose_kernel(float *dst, const float *src, int n0, int n1) {
 
 // ------------------------------------------------------------- launchers
 
-const TileShape kShapes[kNumShapes] = {
-    {2, 4, 1, 1, 64}, {4, 4, 1, 1, 64}, {2, 8, 1, 1, 64}, {2, 16, 1, 1, 64},
-    {1, 8, 1, 1, 64}, {1, 16, 1, 1, 64}, {1, 4, 2, 1, 32}, {2, 4, 2, 1, 32},
-    {1, 8, 2, 1, 32}, {2, 8, 2, 1, 32}, {1, 12, 2, 1, 32}, {2, 2, 2, 1, 32}};
+// the twelve tiles with one step of load lookahead, then with two
+#define BORN_TILES(LA)                                                                     \
+  {2, 4, 1, LA, 64}, {4, 4, 1, LA, 64}, {2, 8, 1, LA, 64}, {2, 16, 1, LA, 64},             \
+      {1, 8, 1, LA, 64}, {1, 16, 1, LA, 64}, {1, 4, 2, LA, 32}, {2, 4, 2, LA, 32},         \
+      {1, 8, 2, LA, 32}, {2, 8, 2, LA, 32}, {1, 12, 2, LA, 32}, {2, 2, 2, LA, 32}
+const TileShape kShapes[kNumShapes] = {BORN_TILES(1), BORN_TILES(2)};
+#undef BORN_TILES
+
+// switch (shape) { case i: return F<..., RY, WY, VX, LA, LX>(ARGS); } over kShapes
+#define BORN_SHAPE_CASES(F, OP, ...)                         \
+  case 0: return F<OP, 2, 4, 1, 1, 64>(__VA_ARGS__);         \
+  case 1: return F<OP, 4, 4, 1, 1, 64>(__VA_ARGS__);         \
+  case 2: return F<OP, 2, 8, 1, 1, 64>(__VA_ARGS__);         \
+  case 3: return F<OP, 2, 16, 1, 1, 64>(__VA_ARGS__);        \
+  case 4: return F<OP, 1, 8, 1, 1, 64>(__VA_ARGS__);         \
+  case 5: return F<OP, 1, 16, 1, 1, 64>(__VA_ARGS__);        \
+  case 6: return F<OP, 1, 4, 2, 1, 32>(__VA_ARGS__);         \
+  case 7: return F<OP, 2, 4, 2, 1, 32>(__VA_ARGS__);         \
+  case 8: return F<OP, 1, 8, 2, 1, 32>(__VA_ARGS__);         \
+  case 9: return F<OP, 2, 8, 2, 1, 32>(__VA_ARGS__);         \
+  case 10: return F<OP, 1, 12, 2, 1, 32>(__VA_ARGS__);       \
+  case 11: return F<OP, 2, 2, 2, 1, 32>(__VA_ARGS__);        \
+  case 12: return F<OP, 2, 4, 1, 2, 64>(__VA_ARGS__);        \
+  case 13: return F<OP, 4, 4, 1, 2, 64>(__VA_ARGS__);        \
+  case 14: return F<OP, 2, 8, 1, 2, 64>(__VA_ARGS__);        \
+  case 15: return F<OP, 2, 16, 1, 2, 64>(__VA_ARGS__);       \
+  case 16: return F<OP, 1, 8, 1, 2, 64>(__VA_ARGS__);        \
+  case 17: return F<OP, 1, 16, 1, 2, 64>(__VA_ARGS__);       \
+  case 18: return F<OP, 1, 4, 2, 2, 32>(__VA_ARGS__);        \
+  case 19: return F<OP, 2, 4, 2, 2, 32>(__VA_ARGS__);        \
+  case 20: return F<OP, 1, 8, 2, 2, 32>(__VA_ARGS__);        \
+  case 21: return F<OP, 2, 8, 2, 2, 32>(__VA_ARGS__);        \
+  case 22: return F<OP, 1, 12, 2, 2, 32>(__VA_ARGS__);       \
+  case 23: return F<OP, 2, 2, 2, 2, 32>(__VA_ARGS__);
 
 static Grid make_grid(const Geometry &geo, const TileShape &ts, int zc) {
   return tile_grid(geo.nx, geo.ny, geo.nz, ts.lx * ts.vx, ts.ry * (kTX * ts.wy / ts.lx),
                    zc > 0 ? zc : 32);
 }
 
-template <class Op, int RY, int WY, int VX, int PZ, int LX>
+template <class Op, int RY, int WY, int VX, int LA, int LX>
 static hipError_t launch_shape(const Geometry &geo, int zc, const Op &op, hipStream_t st) {
   if (VX == 2 && geo.nx % 2) return hipErrorInvalidValue;
-  const Grid g = make_grid(geo, {RY, WY, VX, PZ, LX}, zc);
+  const Grid g = make_grid(geo, {RY, WY, VX, LA, LX}, zc);
   const int nb = g.ntx * g.nty * g.ntz;
-  hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX, PZ, LX>), dim3(nb), dim3(kTX * WY), 0, st,
+  hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX, LA, LX>), dim3(nb), dim3(kTX * WY), 0, st,
                      g, geo.taps, geo.sponge, op);
   return hipGetLastError();
 }
@@ -630,18 +660,7 @@ static hipError_t launch_shape(const Geometry &geo, int zc, const Op &op, hipStr
 template <class Op>
 static hipError_t launch(const Geometry &geo, const Tune &tu, const Op &op, hipStream_t st) {
   switch (tu.shape) {
-    case 0: return launch_shape<Op, 2, 4, 1, 1, 64>(geo, tu.zchunk, op, st);
-    case 1: return launch_shape<Op, 4, 4, 1, 1, 64>(geo, tu.zchunk, op, st);
-    case 2: return launch_shape<Op, 2, 8, 1, 1, 64>(geo, tu.zchunk, op, st);
-    case 3: return launch_shape<Op, 2, 16, 1, 1, 64>(geo, tu.zchunk, op, st);
-    case 4: return launch_shape<Op, 1, 8, 1, 1, 64>(geo, tu.zchunk, op, st);
-    case 5: return launch_shape<Op, 1, 16, 1, 1, 64>(geo, tu.zchunk, op, st);
-    case 6: return launch_shape<Op, 1, 4, 2, 1, 32>(geo, tu.zchunk, op, st);
-    case 7: return launch_shape<Op, 2, 4, 2, 1, 32>(geo, tu.zchunk, op, st);
-    case 8: return launch_shape<Op, 1, 8, 2, 1, 32>(geo, tu.zchunk, op, st);
-    case 9: return launch_shape<Op, 2, 8, 2, 1, 32>(geo, tu.zchunk, op, st);
-    case 10: return launch_shape<Op, 1, 12, 2, 1, 32>(geo, tu.zchunk, op, st);
-    case 11: return launch_shape<Op, 2, 2, 2, 1, 32>(geo, tu.zchunk, op, st);
+    BORN_SHAPE_CASES(launch_shape, Op, geo, tu.zchunk, op, st)
     default: return hipErrorInvalidValue;
   }
 }
@@ -854,7 +873,7 @@ hipError_t launch_adj_finish(const Geometry &geo, float *out, const float *acc,
   return hipGetLastError();
 }
 
-template <class BOp, int RY, int WY, int VX, int PZ, int LX>
+template <class BOp, int RY, int WY, int VX, int LA, int LX>
 static hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdStep &b,
                                      const BOp &op, hipStream_t st) {
   if (b.nshots < 1 || b.nshots > kMaxBatch) return hipErrorInvalidValue;
@@ -870,7 +889,7 @@ static hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdSte
     tiles += g.ntx * g.nty * g.ntz;
   }
   bt.tile0[b.nshots] = tiles;
-  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX, PZ, LX>), dim3(tiles),
+  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX, LA, LX>), dim3(tiles),
                      dim3(kTX * WY), 0, st, bt, taps, op);
   return hipGetLastError();
 }
@@ -879,18 +898,7 @@ template <class BOp>
 static hipError_t launch_batch(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
                                const BOp &op, hipStream_t st) {
   switch (tu.shape) {
-    case 0: return launch_batch_shape<BOp, 2, 4, 1, 1, 64>(taps, tu.zchunk, b, op, st);
-    case 1: return launch_batch_shape<BOp, 4, 4, 1, 1, 64>(taps, tu.zchunk, b, op, st);
-    case 2: return launch_batch_shape<BOp, 2, 8, 1, 1, 64>(taps, tu.zchunk, b, op, st);
-    case 3: return launch_batch_shape<BOp, 2, 16, 1, 1, 64>(taps, tu.zchunk, b, op, st);
-    case 4: return launch_batch_shape<BOp, 1, 8, 1, 1, 64>(taps, tu.zchunk, b, op, st);
-    case 5: return launch_batch_shape<BOp, 1, 16, 1, 1, 64>(taps, tu.zchunk, b, op, st);
-    case 6: return launch_batch_shape<BOp, 1, 4, 2, 1, 32>(taps, tu.zchunk, b, op, st);
-    case 7: return launch_batch_shape<BOp, 2, 4, 2, 1, 32>(taps, tu.zchunk, b, op, st);
-    case 8: return launch_batch_shape<BOp, 1, 8, 2, 1, 32>(taps, tu.zchunk, b, op, st);
-    case 9: return launch_batch_shape<BOp, 2, 8, 2, 1, 32>(taps, tu.zchunk, b, op, st);
-    case 10: return launch_batch_shape<BOp, 1, 12, 2, 1, 32>(taps, tu.zchunk, b, op, st);
-    case 11: return launch_batch_shape<BOp, 2, 2, 2, 1, 32>(taps, tu.zchunk, b, op, st);
+    BORN_SHAPE_CASES(launch_batch_shape, BOp, taps, tu.zchunk, b, op, st)
     default: return hipErrorInvalidValue;
   }
 }

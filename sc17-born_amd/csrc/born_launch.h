@@ -15,13 +15,13 @@ struct Geometry {
 };
 
 // Tile shape: RY rows per thread, WY wavefronts per block, VX cells per lane
-// along x (VX = 2 needs an even nx), PZ planes per step, LX lanes per row
-// (64, or 32: a wavefront covers two rows).  The tile is LX*VX x
-// RY*(64*WY/LX) cells.
+// along x (VX = 2 needs an even nx), LA steps of load lookahead (1 or 2),
+// LX lanes per row (64, or 32: a wavefront covers two rows).  The tile is
+// LX*VX x RY*(64*WY/LX) cells.
 struct TileShape {
-  int ry, wy, vx, pz, lx;
+  int ry, wy, vx, la, lx;
 };
-constexpr int kNumShapes = 12;
+constexpr int kNumShapes = 24;
 extern const TileShape kShapes[kNumShapes];
 
 // Launch configuration of one step kernel: tile shape and z planes per block.

@@ -264,20 +264,19 @@ class Propagator:
             ry, wy, zc, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
             self._chk(self.lib.born_get_tuning(self.h, k, C.byref(ry), C.byref(wy), C.byref(zc),
                                                C.byref(ms)), "tuning")
-            tx, tyy, vx, pz = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            tx, tyy, vx, la = C.c_int(), C.c_int(), C.c_int(), C.c_int()
             self._chk(self.lib.born_get_tile(self.h, k, C.byref(tx), C.byref(tyy), C.byref(vx),
-                                             C.byref(pz)), "tile")
+                                             C.byref(la)), "tile")
             out[name] = {"tile": [tx.value, tyy.value], "rows_per_thread": ry.value,
                          "waves": wy.value, "cells_per_lane": vx.value,
-                         "planes_per_step": pz.value, "zchunk": zc.value,
+                         "lookahead": la.value, "zchunk": zc.value,
                          "ms": round(ms.value, 4)}
         return out
 
-    def set_tuning(self, kernel, rows_per_thread, waves, cells_per_lane, planes_per_step,
-                   zchunk):
+    def set_tuning(self, kernel, rows_per_thread, waves, cells_per_lane, lookahead, zchunk):
         k = self.KERNELS.index(kernel)
         self._chk(self.lib.born_set_tuning(self.h, k, rows_per_thread, waves, cells_per_lane,
-                                           planes_per_step, zchunk), "set_tuning")
+                                           lookahead, zchunk), "set_tuning")
 
     @property
     def stream(self):
