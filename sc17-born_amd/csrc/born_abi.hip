@@ -207,17 +207,41 @@ born::Tune tuning(const born_ctx *c, int kind) {
 // Autotuner: time every tile shape x z chunk for one step kernel and keep
 // the fastest in *tune / *tune_ms.  run(tune) must launch the kernel on
 // scratch buffers only.  Pair shapes (2 cells per lane) need even_nx.
+//
+// z chunks: 32, 64 and 128 planes and, for nzi > 0 interior planes, the
+// lengths near those that split nzi evenly (ceil(nzi / k)), so that no block
+// marches more planes than the others: at 296 planes, chunks of 60 end with
+// a chunk of 56 where chunks of 64 end with one of 40.
+static int zchunk_candidates(int nzi, int (&zcs)[6]) {
+  int n = 0;
+  auto add = [&](int zc) {
+    for (int i = 0; i < n; i++)
+      if (zcs[i] == zc) return;
+    zcs[n++] = zc;
+  };
+  for (int target : {32, 64, 128}) {
+    add(target);
+    if (nzi > 0) {
+      const int k = std::max(1, (int)std::lround((double)nzi / target));
+      add((nzi + k - 1) / k);
+    }
+  }
+  return n;
+}
+
 template <class F>
-int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx, F run) {
+int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx, int nzi,
+                  F run) {
   if (!ctx->autotune || *tune_ms >= 0.f) return BORN_OK;
-  static const int zcs[3] = {32, 64, 128};
+  int zcs[6];
+  const int nzc = zchunk_candidates(nzi, zcs);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   float best = 1e30f;
   born::Tune bt = *tune;
   for (int sh = 0; sh < born::kNumShapes; sh++)
-    for (int zi = 0; zi < 3; zi++) {
+    for (int zi = 0; zi < nzc; zi++) {
       if (born::kShapes[sh].vx == 2 && !even_nx) continue;
       const born::Tune t{sh, ctx->zchunk > 0 ? ctx->zchunk : zcs[zi]};
       if (ctx->zchunk > 0 && zi > 0) continue;
@@ -242,7 +266,8 @@ int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx,
 
 template <class F>
 int autotune(born_ctx *ctx, int kind, F run) {
-  return autotune_into(ctx, &ctx->tune[kind], &ctx->tune_ms[kind], ctx->nx % 2 == 0, run);
+  return autotune_into(ctx, &ctx->tune[kind], &ctx->tune_ms[kind], ctx->nx % 2 == 0,
+                       ctx->nz - 2 * born::kH, run);
 }
 
 // The same for a two-step kernel: its shapes and z chunks that split the
@@ -1787,7 +1812,7 @@ int born_batch_forward(born_batch *b, int jt, int nt, int scatter) {
     HIPCHK(born::launch_batch_ring_init(b->d_rings, b->nshots, ctx->stream));
     born::BatchFwdStep t = batch_step(b, 0, jt, scatter, half);
     t.a.extract = 0;
-    CHK(autotune_into(ctx, &b->tune[kind], &b->tune_ms[kind], b->even_nx,
+    CHK(autotune_into(ctx, &b->tune[kind], &b->tune_ms[kind], b->even_nx, 0,
                       [&](const born::Tune &tu) {
                         return born::launch_batch_fwd_step(taps, tu, t, ctx->stream);
                       }));
