@@ -50,10 +50,9 @@ METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs F
 # HBM bytes per launch of the forward step kernel on the 304^3 workload, from
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of this script
 # (profiles/born_fwd_adj_304cube_hbm_counters.md); reported as roofline.traffic
-# only when the measured workload and kernel are the profiled ones.
-PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 2, 8, 1, 1, 64", "f32"): 1010.3e6,
-                    ("c2", "FwdOp<0, float>, 4, 4, 1, 1, 64", "f32"): 939.6e6,
-                    ("c2", "FwdOp<0, float>, 1, 8, 1, 1, 64", "f32"): 967.5e6}
+# only when the measured workload, kernel and z chunk are the profiled ones.
+PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 4, 4, 1, 1, 64", 60, "f32"): 937.7e6,
+                    ("c2", "FwdOp<0, float>, 2, 8, 1, 1, 64", 60, "f32"): 1007.0e6}
 
 
 def load(name):
@@ -339,7 +338,7 @@ def main():
     fwd_kernel = "FwdOp<0, float>, %d, %d, %d, %d, %d" % (
         ft["rows_per_thread"], ft["waves"], ft["cells_per_lane"], ft["lookahead"],
         64 // ft["cells_per_lane"])
-    traffic = PROFILED_TRAFFIC.get((args.config, fwd_kernel, "f32"))
+    traffic = PROFILED_TRAFFIC.get((args.config, fwd_kernel, ft["zchunk"], "f32"))
     if args.nt or args.shots:
         traffic = None
     out = {

@@ -205,7 +205,8 @@ born::Tune tuning(const born_ctx *c, int kind) {
 }
 
 // Autotuner: time every tile shape x z chunk for one step kernel and keep
-// the fastest in *tune / *tune_ms.  run(tune) must launch the kernel on
+// the fastest in *tune / *tune_ms (ms per launch; the four fastest of a
+// first pass are timed again over more launches).  run(tune) must launch the kernel on
 // scratch buffers only.  Pair shapes (2 cells per lane) need even_nx.
 //
 // z chunks: 32, 64 and 128 planes and, for nzi > 0 interior planes, the
@@ -238,29 +239,44 @@ int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx,
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  float best = 1e30f;
-  born::Tune bt = *tune;
+  // ms per launch of `reps` launches after one untimed launch
+  auto time = [&](const born::Tune &t, int reps, float *ms) -> int {
+    HIPCHK(run(t));
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < reps; r++) HIPCHK(run(t));
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(ms, e0, e1));
+    *ms /= reps;
+    return BORN_OK;
+  };
+  // every candidate over 3 launches, then the 4 fastest again over 10
+  std::vector<std::pair<float, born::Tune>> cand;
   for (int sh = 0; sh < born::kNumShapes; sh++)
     for (int zi = 0; zi < nzc; zi++) {
       if (born::kShapes[sh].vx == 2 && !even_nx) continue;
       const born::Tune t{sh, ctx->zchunk > 0 ? ctx->zchunk : zcs[zi]};
       if (ctx->zchunk > 0 && zi > 0) continue;
-      HIPCHK(run(t));  // warm
-      HIPCHK(hipEventRecord(e0, ctx->stream));
-      for (int r = 0; r < 3; r++) HIPCHK(run(t));
-      HIPCHK(hipEventRecord(e1, ctx->stream));
-      HIPCHK(hipEventSynchronize(e1));
       float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        bt = t;
-      }
+      CHK(time(t, 3, &ms));
+      cand.push_back({ms, t});
     }
+  std::sort(cand.begin(), cand.end(),
+            [](const auto &a, const auto &b) { return a.first < b.first; });
+  float best = 1e30f;
+  born::Tune bt = *tune;
+  for (size_t i = 0; i < cand.size() && i < 4; i++) {
+    float ms = 0.f;
+    CHK(time(cand[i].second, 10, &ms));
+    if (ms < best) {
+      best = ms;
+      bt = cand[i].second;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *tune = bt;
-  *tune_ms = best / 3.f;
+  *tune_ms = best;
   return BORN_OK;
 }
 
