@@ -159,11 +159,11 @@ struct FwdOp {
         p[1][v] = sr[1][v];
       }
     } else {
-      ld_vec<VX>(s0, o, b.io[0], p[0]);
-      ld_vec<VX>(r0, o, b.io[1], p[1]);
+      ld_stream<VX>(s0, o, b.io[0], p[0]);
+      ld_stream<VX>(r0, o, b.io[1], p[1]);
     }
-    ld_vec<VX>(vel, o, 1.f, p[2]);
-    ld_vec<VX>(model, o, 1.f, p[3]);
+    ld_stream<VX>(vel, o, 1.f, p[2]);
+    ld_stream<VX>(model, o, 1.f, p[3]);
   }
   template <int VX>
   __device__ void storev(const Blk2 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
@@ -369,8 +369,8 @@ struct SrecOp {
   }
   template <int VX>
   __device__ void prefetchv(const Blk1 &b, float (&p)[NPRE][VX], uint32_t o) const {
-    ld_vec<VX>(s0, o, b.io, p[0]);
-    ld_vec<VX>(vel, o, 1.f, p[1]);
+    ld_stream<VX>(s0, o, b.io, p[0]);
+    ld_stream<VX>(vel, o, 1.f, p[1]);
   }
   template <int VX>
   __device__ void storev(const Blk1 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
@@ -442,10 +442,10 @@ struct LamOp {
   __device__ int plane(const Blk1 &, int z) const { return ldc(rm.plane_slot, z); }
   template <int VX>
   __device__ void prefetchv(const Blk1 &b, float (&p)[NPRE][VX], uint32_t o) const {
-    ld_vec<VX>(u0, o, b.io, p[0]);
-    ld_vec<VX>(vel, o, 1.f, p[1]);
-    ld_vec<VX>(sig, o, b.sh, p[2]);
-    ld_vec<VX>(acc, o, 1.f, p[3]);
+    ld_stream<VX>(u0, o, b.io, p[0]);
+    ld_stream<VX>(vel, o, 1.f, p[1]);
+    ld_stream<VX>(sig, o, b.sh, p[2]);
+    ld_stream<VX>(acc, o, 1.f, p[3]);
   }
   template <int VX>
   __device__ void storev(const Blk1 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
