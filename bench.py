@@ -51,8 +51,7 @@ METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs F
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of this script
 # (profiles/born_fwd_adj_304cube_hbm_counters.md); reported as roofline.traffic
 # only when the measured workload, kernel and z chunk are the profiled ones.
-PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 4, 4, 1, 1, 64", 60, "f32"): 937.7e6,
-                    ("c2", "FwdOp<0, float>, 2, 8, 1, 1, 64", 60, "f32"): 1007.0e6}
+PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 4, 4, 1, 1, 64", 60, "f32"): 980.8e6}
 
 
 def load(name):
