@@ -86,20 +86,35 @@ __device__ __forceinline__ const T &at(const T *p, uint32_t o) {
 // A vector memory instruction that reads a scalar register written by a
 // vector instruction (v_readlane, as in a reload of a spilled scalar) needs
 // 5 wait states before it, which the compiler does not insert for inline
-// asm: each store starts with s_nop 4.  The stores carry the non-temporal
-// hint (nt): no step reads a new level back before the next launch, and the
-// hint took 6 % off the FP32 forward step at 304^3.
+// asm: each store starts with s_nop 4.  NT adds the non-temporal hint: no
+// step reads a new level back before the next launch, and the hint took 6 %
+// off the FP32 forward step at 304^3.
+template <bool NT>
 __device__ __forceinline__ void st_b16(void *p, uint32_t off, unsigned short v) {
-  asm volatile("s_nop 4\n\tglobal_store_short %0, %1, %2 nt"
-               : : "v"(off), "v"((uint32_t)v), "s"(p) : "memory");
+  if constexpr (NT)
+    asm volatile("s_nop 4\n\tglobal_store_short %0, %1, %2 nt"
+                 : : "v"(off), "v"((uint32_t)v), "s"(p) : "memory");
+  else
+    asm volatile("s_nop 4\n\tglobal_store_short %0, %1, %2"
+                 : : "v"(off), "v"((uint32_t)v), "s"(p) : "memory");
 }
+template <bool NT>
 __device__ __forceinline__ void st_b32(void *p, uint32_t off, uint32_t v) {
-  asm volatile("s_nop 4\n\tglobal_store_dword %0, %1, %2 nt"
-               : : "v"(off), "v"(v), "s"(p) : "memory");
+  if constexpr (NT)
+    asm volatile("s_nop 4\n\tglobal_store_dword %0, %1, %2 nt"
+                 : : "v"(off), "v"(v), "s"(p) : "memory");
+  else
+    asm volatile("s_nop 4\n\tglobal_store_dword %0, %1, %2"
+                 : : "v"(off), "v"(v), "s"(p) : "memory");
 }
+template <bool NT>
 __device__ __forceinline__ void st_b64(void *p, uint32_t off, uint64_t v) {
-  asm volatile("s_nop 4\n\tglobal_store_dwordx2 %0, %1, %2 nt"
-               : : "v"(off), "v"(v), "s"(p) : "memory");
+  if constexpr (NT)
+    asm volatile("s_nop 4\n\tglobal_store_dwordx2 %0, %1, %2 nt"
+                 : : "v"(off), "v"(v), "s"(p) : "memory");
+  else
+    asm volatile("s_nop 4\n\tglobal_store_dwordx2 %0, %1, %2"
+                 : : "v"(off), "v"(v), "s"(p) : "memory");
 }
 
 // Block barrier for LDS: the block's LDS accesses complete, then s_barrier.
@@ -294,19 +309,19 @@ __device__ __forceinline__ void ld_pair(const __half2 *p, uint32_t o, const floa
     v[1][1] = __high2float(b) * inv[1];
   }
 }
-template <int VX>
+template <int VX, bool NT = true>
 __device__ __forceinline__ void st_pair(__half2 *p, uint32_t o, const float (&a)[VX],
                                         const float (&b)[VX], const float (&scale)[2]) {
   const __half2 c0 = __floats2half2_rn(sat_half(a[0] * scale[0]), sat_half(b[0] * scale[1]));
   uint32_t u0;
   __builtin_memcpy(&u0, &c0, 4);
   if constexpr (VX == 1) {
-    st_b32(p, o * 4u, u0);
+    st_b32<NT>(p, o * 4u, u0);
   } else {
     const __half2 c1 = __floats2half2_rn(sat_half(a[1] * scale[0]), sat_half(b[1] * scale[1]));
     uint32_t u1;
     __builtin_memcpy(&u1, &c1, 4);
-    st_b64(p, o * 4u, (uint64_t)u1 << 32 | u0);
+    st_b64<NT>(p, o * 4u, (uint64_t)u1 << 32 | u0);
   }
 }
 
@@ -356,23 +371,23 @@ __device__ __forceinline__ void ld_stream(const __half *p, uint32_t o, float inv
   ld_vec<VX>(p, o, inv, v);
 }
 
-template <int VX>
+template <int VX, bool NT = true>
 __device__ __forceinline__ void st_vec(float *p, uint32_t o, const float (&v)[VX], float) {
   if constexpr (VX == 1)
-    st_b32(p, o * 4u, __float_as_uint(v[0]));
+    st_b32<NT>(p, o * 4u, __float_as_uint(v[0]));
   else
-    st_b64(p, o * 4u, (uint64_t)__float_as_uint(v[1]) << 32 | __float_as_uint(v[0]));
+    st_b64<NT>(p, o * 4u, (uint64_t)__float_as_uint(v[1]) << 32 | __float_as_uint(v[0]));
 }
-template <int VX>
+template <int VX, bool NT = true>
 __device__ __forceinline__ void st_vec(__half *p, uint32_t o, const float (&v)[VX],
                                        float scale) {
   if constexpr (VX == 1) {
-    st_b16(p, o * 2u, __half_as_ushort(__float2half_rn(sat_half(v[0] * scale))));
+    st_b16<NT>(p, o * 2u, __half_as_ushort(__float2half_rn(sat_half(v[0] * scale))));
   } else {
     const __half2 c = __floats2half2_rn(sat_half(v[0] * scale), sat_half(v[1] * scale));
     uint32_t u;
     __builtin_memcpy(&u, &c, 4);
-    st_b32(p, o * 2u, u);
+    st_b32<NT>(p, o * 2u, u);
   }
 }
 
