@@ -453,14 +453,22 @@ struct LamOp {
   template <int VX>
   __device__ void prefetchv(const Blk1 &b, float (&p)[NPRE][VX], uint32_t o) const {
     ld_stream<VX>(u0, o, b.io, p[0]);
-    ld_stream<VX>(vel, o, 1.f, p[1]);
     ld_stream<VX>(sig, o, b.sh, p[2]);
-    ld_stream<VX>(acc, o, 1.f, p[3]);
+    // FP32: velocity and image without the non-temporal hint (read, and the
+    // image written, every step: 143 against 155 us per step at 304^3);
+    // the FP16 step is faster with it
+    if constexpr (H) {
+      ld_stream<VX>(vel, o, 1.f, p[1]);
+      ld_stream<VX>(acc, o, 1.f, p[3]);
+    } else {
+      ld_vec<VX>(vel, o, 1.f, p[1]);
+      ld_vec<VX>(acc, o, 1.f, p[3]);
+    }
   }
   template <int VX>
   __device__ void storev(const Blk1 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
     st_vec<VX>(u0, o, out[0], b.sn);
-    st_vec<VX>(acc, o, out[1], 1.f);
+    st_vec<VX, H>(acc, o, out[1], 1.f);
   }
   __device__ void cell(const Blk1 &, int slot, const CellArgs &c, const float *lap,
                        const float *ctr, const float *p, float (&out)[NOUT],
