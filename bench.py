@@ -51,7 +51,7 @@ METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs F
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of this script
 # (profiles/born_fwd_adj_304cube_hbm_counters.md); reported as roofline.traffic
 # only when the measured workload, kernel and z chunk are the profiled ones.
-PROFILED_TRAFFIC = {("c2", "FwdOp<0, float>, 4, 4, 1, 1, 64", 60, "f32"): 980.8e6}
+PROFILED_TRAFFIC = {("c2", "FwdOp<0, float, true>, 4, 4, 1, 1, 64", 60, "f32"): 980.8e6}
 
 
 def load(name):
@@ -334,7 +334,7 @@ def main():
     achieved = fwd_alg_bytes / fwd_launch_s / 1e9
     alg_gbs = born_bytes("f32", nt, recomputed) * cell_steps / elapsed / 1e9
     ft = tuning["fwd"]
-    fwd_kernel = "FwdOp<0, float>, %d, %d, %d, %d, %d" % (
+    fwd_kernel = "FwdOp<0, float, true>, %d, %d, %d, %d, %d" % (
         ft["rows_per_thread"], ft["waves"], ft["cells_per_lane"], ft["lookahead"],
         64 // ft["cells_per_lane"])
     traffic = PROFILED_TRAFFIC.get((args.config, fwd_kernel, ft["zchunk"], "f32"))
