@@ -37,7 +37,7 @@ def files(tmp_path_factory):
     img[10:, :, :] = rng.standard_normal((nz - 10, ny, nx)).astype(np.float32) * 20
     hcube.write_json(str(d / "velocity.json"), vel, [(nx, 0.0, h), (ny, 0.0, h), (nz, 0.0, h)])
     hcube.write_json(str(d / "image.json"), img, [(nx, 0.0, h), (ny, 0.0, h), (nz, 0.0, h)])
-    run("Wavelet", "out=wavelet.json", "nt=100", "dt=.001", "fund=12", cwd=d)
+    run("Wavelet", "out=wavelet.json", "nt=60", "dt=.002", "fund=12", cwd=d)
     # 90 samples at 2 ms, 10 x 8 receivers every 30 m, 2 x 1 shots
     nt, dt = 90, 0.002
     axes = [(nt, 0.0, dt), (10, 60.0, 30.0), (8, 60.0, 30.0), (2, 150.0, 120.0), (1, 200.0, 10.0)]
