@@ -72,6 +72,7 @@ struct FwdOp {
   // FP16 levels hold s and r as one __half2 per cell (sr1: level k, sr0:
   // level k-1, srn: level k+1, the same buffer as sr0 in place)
   static constexpr bool PAIRED = H;
+  static constexpr bool Q2 = H;  // steps in pairs (march_tile)
   static constexpr bool DAMP = true;
   const T *__restrict__ s1;
   const T *__restrict__ r1;
@@ -221,6 +222,7 @@ struct FwdOp {
 struct SrcOp {
   static constexpr int NF = 1, NPRE = 2, NOUT = 1, NMAX = 0;
   static constexpr bool PAIRED = false;
+  static constexpr bool Q2 = false;
   static constexpr bool DAMP = false;
   const float *__restrict__ s1;
   float *__restrict__ s0;
@@ -259,6 +261,7 @@ struct SrcOp {
 struct RadjOp {
   static constexpr int NF = 2, NPRE = 5, NOUT = 3, NMAX = 0;
   static constexpr bool PAIRED = false;
+  static constexpr bool Q2 = false;
   static constexpr bool DAMP = true;
   const float *__restrict__ s1;
   const float *__restrict__ r1;
@@ -332,6 +335,7 @@ struct SrecOp {
   static constexpr bool H = kHalf<T>;
   static constexpr int NF = 1, NPRE = 2, NOUT = 2, NMAX = H ? 1 : 0;
   static constexpr bool PAIRED = false;
+  static constexpr bool Q2 = H;  // steps in pairs (march_tile)
   static constexpr bool DAMP = true;
   const T *__restrict__ s1;
   // old s level (read) and new s level (written): the same buffer, or with
@@ -407,6 +411,7 @@ struct LamOp {
   static constexpr bool H = kHalf<T>;
   static constexpr int NF = 1, NPRE = 4, NOUT = 2, NMAX = H ? 1 : 0;
   static constexpr bool PAIRED = false;
+  static constexpr bool Q2 = H;  // steps in pairs (march_tile)
   static constexpr bool DAMP = true;
   const T *__restrict__ u1;
   T *__restrict__ u0;
