@@ -384,7 +384,11 @@ struct SrecOp {
   template <int VX>
   __device__ void prefetchv(const Blk1 &b, float (&p)[NPRE][VX], uint32_t o) const {
     ld_stream<VX>(s0, o, b.io, p[0]);
-    ld_stream<VX>(vel, o, 1.f, p[1]);
+    // FP32: the velocity without the hint (114 -> 103 us per step at 304^3)
+    if constexpr (H)
+      ld_stream<VX>(vel, o, 1.f, p[1]);
+    else
+      ld_vec<VX>(vel, o, 1.f, p[1]);
   }
   template <int VX>
   __device__ void storev(const Blk1 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
