@@ -209,24 +209,26 @@ born::Tune tuning(const born_ctx *c, int kind) {
 // first pass are timed again over more launches).  run(tune) must launch the kernel on
 // scratch buffers only.  Pair shapes (2 cells per lane) need even_nx.
 //
-// z chunks: 32, 64 and 128 planes and, for nzi > 0 interior planes, the
-// lengths near those that split nzi evenly (ceil(nzi / k)), so that no block
-// marches more planes than the others: at 296 planes, chunks of 60 end with
-// a chunk of 56 where chunks of 64 end with one of 40.
-static int zchunk_candidates(int nzi, int (&zcs)[6]) {
+// z chunks: 32, 64 and 128 planes and, for nzi > 0 interior planes, every
+// length that splits nzi evenly into k = 1..16 chunks (ceil(nzi / k), at
+// least 16 planes), so that no block marches more planes than the others
+// (at 296 planes, chunks of 60 end with one of 56 where chunks of 64 end
+// with one of 40) and the number of blocks can match what the CUs hold at
+// once.
+constexpr int kMaxZChunks = 19;
+static int zchunk_candidates(int nzi, int (&zcs)[kMaxZChunks]) {
   int n = 0;
   auto add = [&](int zc) {
     for (int i = 0; i < n; i++)
       if (zcs[i] == zc) return;
-    zcs[n++] = zc;
+    if (n < kMaxZChunks) zcs[n++] = zc;
   };
-  for (int target : {32, 64, 128}) {
-    add(target);
-    if (nzi > 0) {
-      const int k = std::max(1, (int)std::lround((double)nzi / target));
-      add((nzi + k - 1) / k);
+  for (int target : {32, 64, 128}) add(target);
+  if (nzi > 0)
+    for (int k = 1; k <= 16; k++) {
+      const int zc = (nzi + k - 1) / k;
+      if (zc >= 16) add(zc);
     }
-  }
   return n;
 }
 
@@ -234,7 +236,7 @@ template <class F>
 int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx, int nzi,
                   F run) {
   if (!ctx->autotune || *tune_ms >= 0.f) return BORN_OK;
-  int zcs[6];
+  int zcs[kMaxZChunks];
   const int nzc = zchunk_candidates(nzi, zcs);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
