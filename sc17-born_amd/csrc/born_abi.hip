@@ -903,7 +903,22 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     t.tr = traces; t.tab = ctx->d_tab_d; t.extract = 0; t.scatter = scatter;
     t.h = half_state(ctx, 0);
     t.rg = ctx->d_ring + RING_T0; t.hscale = ctx->d_ring + HS_T;
+    // The buffers the steps below use.  With the s levels in the kept
+    // history each timed launch takes the next three history levels, as
+    // the steps do (levels that the steps overwrite); the same three levels
+    // launch after launch time up to 20 % faster than the steps run, and
+    // pick another tile.  Else the history level written beside the
+    // in-place s.
+    const bool rotate = hist && scatter == 0 && !half && nt >= 3;
+    if (!rotate) t.hist = hist;
+    int lev = 0;
     CHK(autotune(ctx, kind, [&](const born::Tune &tu) {
+      if (rotate) {
+        t.s0 = nth(hist, lev, n, esz);
+        t.s1 = nth(hist, lev + 1, n, esz);
+        t.s_new = nth(hist, lev + 2, n, esz);
+        lev = (lev + 1) % (nt - 2);
+      }
       return born::launch_fwd_step(geo, tu, t, ctx->stream); }));
   }
   const born::Tune tu = tuning(ctx, kind);
