@@ -206,7 +206,7 @@ born::Tune tuning(const born_ctx *c, int kind) {
 
 // Autotuner: time every tile shape x z chunk for one step kernel and keep
 // the fastest in *tune / *tune_ms (ms per launch; the four fastest of a
-// first pass are timed again over more launches).  run(tune) must launch the kernel on
+// first pass are timed again, round-robin, over more launches).  run(tune) must launch the kernel on
 // scratch buffers only.  Pair shapes (2 cells per lane) need even_nx.
 //
 // z chunks: 32, 64 and 128 planes and, for nzi > 0 interior planes, every
@@ -252,7 +252,7 @@ int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx,
     *ms /= reps;
     return BORN_OK;
   };
-  // every candidate over 3 launches, then the 4 fastest again over 10
+  // every candidate over 3 launches
   std::vector<std::pair<float, born::Tune>> cand;
   for (int sh = 0; sh < born::kNumShapes; sh++)
     for (int zi = 0; zi < nzc; zi++) {
@@ -265,16 +265,23 @@ int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx,
     }
   std::sort(cand.begin(), cand.end(),
             [](const auto &a, const auto &b) { return a.first < b.first; });
+  // the four fastest again, round-robin over three rounds of 30 launches
+  // (a drift of the clocks during the search then favours none of them)
+  const size_t top = std::min<size_t>(cand.size(), 4);
+  float sum[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int round = 0; round < 3; round++)
+    for (size_t i = 0; i < top; i++) {
+      float ms = 0.f;
+      CHK(time(cand[i].second, 30, &ms));
+      sum[i] += ms / 3.f;
+    }
   float best = 1e30f;
   born::Tune bt = *tune;
-  for (size_t i = 0; i < cand.size() && i < 4; i++) {
-    float ms = 0.f;
-    CHK(time(cand[i].second, 10, &ms));
-    if (ms < best) {
-      best = ms;
+  for (size_t i = 0; i < top; i++)
+    if (sum[i] < best) {
+      best = sum[i];
       bt = cand[i].second;
     }
-  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *tune = bt;
