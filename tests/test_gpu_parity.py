@@ -1,7 +1,8 @@
 """HIP path against the CPU oracle (reference semantics of base/cpu_prop.cc).
 
-Tolerances: the north star asks for relative L2 <= 1e-4 on data and image
-against the reference CPU path; the kernels use fused multiply-adds where the
+Tolerances: relative L2 <= 1e-4 on data and image against the reference
+CPU path, the bar for floating-point parity; the kernels use fused
+multiply-adds where the
 reference rounds twice, so parity is a tolerance, not bit equality.  Integer
 work (receiver maps, checkpoint plans) must match exactly.
 """
