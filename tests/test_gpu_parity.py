@@ -2,9 +2,9 @@
 
 Tolerances: relative L2 <= 1e-4 on data and image against the reference
 CPU path, the bar for floating-point parity; the kernels use fused
-multiply-adds where the
-reference rounds twice, so parity is a tolerance, not bit equality.  Integer
-work (receiver maps, checkpoint plans) must match exactly.
+multiply-adds where the reference rounds twice, so parity is a tolerance,
+not bit equality.  Integer work (receiver maps, checkpoint plans) must
+match exactly.
 """
 import numpy as np
 import pytest
