@@ -104,7 +104,7 @@ def lib():
 
 
 def ref_lib():
-    """The reference's sinc_bob.cc compiled from /root/reference (or None)."""
+    """The reference's base/sinc_bob.cc compiled into oracle/_ref (or None)."""
     global _REF
     if _REF is None:
         path = os.path.join(_HERE, "_ref", "libsinc_ref.so")

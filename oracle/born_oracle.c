@@ -19,7 +19,7 @@
  *                  at the far face, untouched centre) and cell counts per edge.
  *   inject/extract brute force; jt=1 reduces to a single tap.
  *   sinc table     bit-exact against the reference sinc_bob.cc compiled from
- *                  /root/reference into oracle/_ref (jt = 1..8).
+ *                  its own source into oracle/_ref (jt = 1..8).
  *   loops          linearity in the model, zero model -> zero data, time
  *                  reversibility of the undamped background, and the exact
  *                  adjoint's dot test against the forward (FP64).

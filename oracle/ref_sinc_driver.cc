@@ -1,5 +1,5 @@
-// Driver for the reference sinc_bob.cc compiled where it lies under
-// /root/reference (see oracle/Makefile).  TEST INFRASTRUCTURE: it exposes the
+// Driver for the reference's base/sinc_bob.cc, compiled where that source
+// lies (the directory is REF in oracle/Makefile).  TEST INFRASTRUCTURE: it exposes the
 // reference's table generator (base/sinc_bob.cc:4-16) through a C symbol so
 // the tests can pin the oracle's restatement bit for bit.
 #include "sinc_bob.h"

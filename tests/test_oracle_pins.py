@@ -21,7 +21,7 @@ def space(d1, d2, d3, nx, ny, nz, width=None):
 # ---------------------------------------------------------------- sinc_bob
 @pytest.mark.parametrize("jt", [1, 2, 3, 4, 5, 7, 8])
 def test_sinc_table_bit_exact_against_compiled_reference(jt):
-    """base/sinc_bob.cc compiled from /root/reference into oracle/_ref."""
+    """The reference's base/sinc_bob.cc, compiled from its source into oracle/_ref."""
     ref = oracle.ref_sinc_table(jt)
     if ref is None:
         pytest.skip("oracle/_ref not built (reference sources absent)")
