@@ -119,7 +119,10 @@ int born_set_sponge(born_ctx *ctx, int width, float bcB);
 
 /* Exact adjoint of born_rtm_forward (scatter 0) or of the linearised damped
  * wave equation in V = v^2 dt^2 (scatter 1): img (n1*n2*n3) += F^T rec, with
- * rec [nrec][nt_big].  The background is rebuilt from HBM checkpoints. */
+ * rec [nrec][nt_big].  The background is rebuilt from HBM checkpoints.
+ * The adjoint state is carried as V B lambda and the image divided by V B,
+ * so velocity 2 and every sponge factor must be > 0 in the stencil interior;
+ * otherwise the call returns BORN_ERR_UNSUPPORTED (the device form too). */
 int born_adjoint_exact(born_ctx *ctx, int n1, int n2, int n3, int jt,
                        const float *rec, float *img, int nt, int scatter);
 
@@ -205,8 +208,11 @@ void *born_get_stream(const born_ctx *ctx);
  * domain, velocity, model, source and receivers, advance together: one
  * launch per time step covers every shot's tiles, and the time loop is
  * captured once in a HIP graph and replayed.  The spacing is shared; the
- * sinc tables, sponge, storage (born_set_storage) and stream come from ctx,
- * which must outlive the batch.  Every shot's nx * ny * nz < 2^30. */
+ * sinc tables, storage (born_set_storage) and stream come from ctx at each
+ * call (a new sinc table makes the next call capture the loop again), and
+ * ctx must outlive the batch.  The sponge (born_set_sponge) is copied when
+ * the batch is created: a later born_set_sponge does not change the batch.
+ * Every shot's nx * ny * nz < 2^30. */
 typedef struct born_batch born_batch;
 typedef struct {
   int nx, ny, nz;        /* the shot's domain, x fastest */

@@ -52,55 +52,75 @@ def lib():
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
-        L = C.CDLL(path)
-        P = C.POINTER(OrcProp)
-        sig = {
-            "orc_create_space": (None, [P, C.c_float, C.c_float, C.c_float, C.c_float,
-                                        C.c_float, C.c_float, C.c_int, C.c_int, C.c_int]),
-            "orc_set_sponge": (None, [P, C.c_int, C.c_float]),
-            "orc_prop_step": (None, [P, f32p, f32p, f32p]),
-            "orc_damp": (None, [P, f32p, f32p]),
-            "orc_inject_source": (None, [P, C.c_int, C.c_int, f32p]),
-            "orc_inject_receivers": (None, [P, C.c_int, C.c_int, f32p]),
-            "orc_data_extract": (None, [P, C.c_int, C.c_int, f32p]),
-            "orc_image_add": (None, [P, f32p, f32p, f32p]),
-            "orc_image_condition": (None, [P, f32p, f32p, f32p]),
-            "orc_rtm_forward": (None, [P, C.c_int, f32p, C.c_int]),
-            "orc_source_prop": (None, [P, f32p, f32p, C.c_int, C.c_int]),
-            "orc_rtm_adjoint": (None, [P, f32p, f32p, f32p, C.c_int]),
-            "orc_born_forward_d": (None, [P, C.c_int, C.c_int, f32p, C.c_int, f64p]),
-            "orc_exact_adjoint_d": (None, [P, C.c_int, C.c_int, f64p, C.c_int, f64p]),
-            "orc_exact_adjoint_f": (None, [P, C.c_int, C.c_int, f32p, C.c_int, f32p]),
-            "orc_nonlinear_d": (None, [P, C.c_int, f64p, C.c_int, f64p]),
-            "orc_sinc_table": (None, [C.c_int, C.c_int, f32p]),
-            "orc_calc_stability": (None, [C.c_float, C.c_float, f32p, C.c_int, C.c_int,
-                                          C.POINTER(C.c_float), C.POINTER(C.c_int)]),
-            "orc_padded_size": (C.c_int, [C.c_int, C.c_int, C.c_int]),
-            "orc_wavelet_deriv": (None, [C.c_int, C.c_float, f32p, f32p]),
-            "orc_source_func": (None, [C.c_int, C.c_int] + [C.c_float] * 9 +
-                                [f32p, C.c_int, C.c_int, i32p, f32p]),
-            "orc_rand_subspace": (C.c_float, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
-                                              C.c_int, C.c_int, C.c_int] + [C.c_float] * 6 +
-                                  [f32p, C.c_int, C.c_int, C.c_int] + [C.c_float] * 6 + [f32p]),
-            "orc_receiver_locs": (None, [C.c_int, C.c_int] + [C.c_float] * 5 +
-                                  [C.c_int, C.c_int] + [C.c_float] * 6 + [i32p]),
-            "orc_map_data_forward": (None, [C.c_int] * 4 + [f32p, f32p]),
-            "orc_map_data_adjoint": (None, [C.c_int] * 4 + [f32p, f32p]),
-            "orc_deriv_forward": (None, [C.c_int] * 3 + [C.c_float, f32p, f32p]),
-            "orc_deriv_adjoint": (None, [C.c_int] * 3 + [C.c_float, f32p, f32p]),
-            "orc_laplac_forward": (None, [C.c_int] * 3 + [f32p, f32p]),
-            "orc_laplac_adjoint": (None, [C.c_int] * 3 + [f32p, f32p]),
-            "orc_extract_sub": (None, [C.c_int] * 3 + [C.c_float] * 6 + [f32p] +
-                                [C.c_int] * 3 + [C.c_float] * 3 + [f32p]),
-            "orc_add_image": (None, [C.c_int] * 3 + [C.c_float] * 6 + [f32p] +
-                              [C.c_int] * 3 + [C.c_float] * 3 + [f32p]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _LIB = L
+        _LIB = _bind(C.CDLL(path))
     return _LIB
+
+
+_VARIANTS = {}
+
+
+def variant_lib(name):
+    """The same C source built for speed on the host that loads it
+    (oracle/Makefile target `native`: "native" = -O3 -march=native, "ofast" =
+    -Ofast -march=native, the reference's CMakeLists.txt:10 flags).  Timing
+    only: FP contraction changes the rounding, so parity checks use lib()."""
+    if name not in _VARIANTS:
+        path = os.path.join(_HERE, "_native", "liboracle_%s.so" % name)
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built (make -C oracle native)" % path)
+        _VARIANTS[name] = _bind(C.CDLL(path))
+    return _VARIANTS[name]
+
+
+def _bind(L):
+    """Set the ctypes signatures of every exported oracle function."""
+    P = C.POINTER(OrcProp)
+    sig = {
+        "orc_create_space": (None, [P, C.c_float, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_float, C.c_int, C.c_int, C.c_int]),
+        "orc_set_sponge": (None, [P, C.c_int, C.c_float]),
+        "orc_prop_step": (None, [P, f32p, f32p, f32p]),
+        "orc_damp": (None, [P, f32p, f32p]),
+        "orc_inject_source": (None, [P, C.c_int, C.c_int, f32p]),
+        "orc_inject_receivers": (None, [P, C.c_int, C.c_int, f32p]),
+        "orc_data_extract": (None, [P, C.c_int, C.c_int, f32p]),
+        "orc_image_add": (None, [P, f32p, f32p, f32p]),
+        "orc_image_condition": (None, [P, f32p, f32p, f32p]),
+        "orc_rtm_forward": (None, [P, C.c_int, f32p, C.c_int]),
+        "orc_source_prop": (None, [P, f32p, f32p, C.c_int, C.c_int]),
+        "orc_rtm_adjoint": (None, [P, f32p, f32p, f32p, C.c_int]),
+        "orc_born_forward_d": (None, [P, C.c_int, C.c_int, f32p, C.c_int, f64p]),
+        "orc_exact_adjoint_d": (None, [P, C.c_int, C.c_int, f64p, C.c_int, f64p]),
+        "orc_exact_adjoint_f": (None, [P, C.c_int, C.c_int, f32p, C.c_int, f32p]),
+        "orc_nonlinear_d": (None, [P, C.c_int, f64p, C.c_int, f64p]),
+        "orc_sinc_table": (None, [C.c_int, C.c_int, f32p]),
+        "orc_calc_stability": (None, [C.c_float, C.c_float, f32p, C.c_int, C.c_int,
+                                      C.POINTER(C.c_float), C.POINTER(C.c_int)]),
+        "orc_padded_size": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+        "orc_wavelet_deriv": (None, [C.c_int, C.c_float, f32p, f32p]),
+        "orc_source_func": (None, [C.c_int, C.c_int] + [C.c_float] * 9 +
+                            [f32p, C.c_int, C.c_int, i32p, f32p]),
+        "orc_rand_subspace": (C.c_float, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                          C.c_int, C.c_int, C.c_int] + [C.c_float] * 6 +
+                              [f32p, C.c_int, C.c_int, C.c_int] + [C.c_float] * 6 + [f32p]),
+        "orc_receiver_locs": (None, [C.c_int, C.c_int] + [C.c_float] * 5 +
+                              [C.c_int, C.c_int] + [C.c_float] * 6 + [i32p]),
+        "orc_map_data_forward": (None, [C.c_int] * 4 + [f32p, f32p]),
+        "orc_map_data_adjoint": (None, [C.c_int] * 4 + [f32p, f32p]),
+        "orc_deriv_forward": (None, [C.c_int] * 3 + [C.c_float, f32p, f32p]),
+        "orc_deriv_adjoint": (None, [C.c_int] * 3 + [C.c_float, f32p, f32p]),
+        "orc_laplac_forward": (None, [C.c_int] * 3 + [f32p, f32p]),
+        "orc_laplac_adjoint": (None, [C.c_int] * 3 + [f32p, f32p]),
+        "orc_extract_sub": (None, [C.c_int] * 3 + [C.c_float] * 6 + [f32p] +
+                            [C.c_int] * 3 + [C.c_float] * 3 + [f32p]),
+        "orc_add_image": (None, [C.c_int] * 3 + [C.c_float] * 6 + [f32p] +
+                          [C.c_int] * 3 + [C.c_float] * 3 + [f32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
 
 
 def ref_lib():
@@ -145,10 +165,11 @@ class Prop:
     transferReceiverFunc, then rtmForward or sourceProp + rtmAdjoint).
     Arrays handed over are kept alive here, as the reference aliases them."""
 
-    def __init__(self, nthreads=0):
+    def __init__(self, nthreads=0, variant=None):
         self.s = OrcProp()
         self.s.nthreads = nthreads or (os.cpu_count() or 1)
         self._keep = {}
+        self.L = lib() if variant is None else variant_lib(variant)
 
     @property
     def ptr(self):
@@ -160,10 +181,10 @@ class Prop:
         return a.ctypes.data
 
     def create_space(self, d1, d2, d3, nx, ny, nz, bc_a=50.0, bc_b=0.0005, bc_y=0.0005):
-        lib().orc_create_space(self.ptr, d1, d2, d3, bc_a, bc_b, bc_y, nx, ny, nz)
+        self.L.orc_create_space(self.ptr, d1, d2, d3, bc_a, bc_b, bc_y, nx, ny, nz)
 
     def set_sponge(self, width, bcB=0.0005):
-        lib().orc_set_sponge(self.ptr, width, bcB)
+        self.L.orc_set_sponge(self.ptr, width, bcB)
 
     @property
     def n(self):
@@ -205,38 +226,38 @@ class Prop:
         self.s.rec = rec.ctypes.data
 
     def rtm_forward(self, jt, img, nt):
-        lib().orc_rtm_forward(self.ptr, jt, f32(np.ravel(img)), nt)
+        self.L.orc_rtm_forward(self.ptr, jt, f32(np.ravel(img)), nt)
 
     def source_prop(self, p0, p1, jts, nt):
-        lib().orc_source_prop(self.ptr, p0, p1, jts, nt)
+        self.L.orc_source_prop(self.ptr, p0, p1, jts, nt)
 
     def rtm_adjoint(self, p0, p1, img, nt):
-        lib().orc_rtm_adjoint(self.ptr, p0, p1, img, nt)
+        self.L.orc_rtm_adjoint(self.ptr, p0, p1, img, nt)
 
     def prop_step(self, p0, p1, vel):
-        lib().orc_prop_step(self.ptr, p0, p1, vel)
+        self.L.orc_prop_step(self.ptr, p0, p1, vel)
 
     def damp(self, p0, p1):
-        lib().orc_damp(self.ptr, p0, p1)
+        self.L.orc_damp(self.ptr, p0, p1)
 
     def born_forward_d(self, jt, img, nt, scatter=0):
         out = np.zeros(self.s.nrec * self.s.nt_rec, np.float64)
-        lib().orc_born_forward_d(self.ptr, jt, scatter, f32(np.ravel(img)), nt, out)
+        self.L.orc_born_forward_d(self.ptr, jt, scatter, f32(np.ravel(img)), nt, out)
         return out.reshape(self.s.nrec, self.s.nt_rec)
 
     def exact_adjoint_d(self, jt, rec, nt, scatter=0):
         out = np.zeros(self.n, np.float64)
-        lib().orc_exact_adjoint_d(self.ptr, jt, scatter,
+        self.L.orc_exact_adjoint_d(self.ptr, jt, scatter,
                                   np.ascontiguousarray(rec, np.float64).ravel(), nt, out)
         return out.reshape(self.shape)
 
     def exact_adjoint_f(self, jt, rec, nt, scatter=0):
         out = np.zeros(self.n, np.float32)
-        lib().orc_exact_adjoint_f(self.ptr, jt, scatter, f32(rec).ravel(), nt, out)
+        self.L.orc_exact_adjoint_f(self.ptr, jt, scatter, f32(rec).ravel(), nt, out)
         return out.reshape(self.shape)
 
     def nonlinear_d(self, jt, vel2dt2, nt):
         out = np.zeros(self.s.nrec * self.s.nt_rec, np.float64)
-        lib().orc_nonlinear_d(self.ptr, jt, np.ascontiguousarray(np.ravel(vel2dt2), np.float64),
+        self.L.orc_nonlinear_d(self.ptr, jt, np.ascontiguousarray(np.ravel(vel2dt2), np.float64),
                               nt, out)
         return out.reshape(self.s.nrec, self.s.nt_rec)

@@ -20,13 +20,19 @@
  *   inject/extract brute force; jt=1 reduces to a single tap.
  *   sinc table     bit-exact against the reference sinc_bob.cc compiled from
  *                  its own source into oracle/_ref (jt = 1..8).
- *   loops          linearity in the model, zero model -> zero data, time
- *                  reversibility of the undamped background, and the exact
- *                  adjoint's dot test against the forward (FP64).
+ *   loops          impulse responses derived by hand from the reference's
+ *                  step lists (tests/test_oracle_kat.py: rtmForward data,
+ *                  sourceProp levels and rtmAdjoint image for nt <= 3, one
+ *                  source and receiver, exact one-tap tables); linearity in
+ *                  the model, time reversibility of the undamped background,
+ *                  and the exact adjoint's dot test against the forward (FP64).
  *   rand_subspace  nrand = 0 against the closed form (v^2 dt^2 of the nearest
- *                  lower model cell); the random band against statistical
- *                  bounds only: the reference draws rand() inside a parallel
- *                  loop, so its band is not reproducible (parity unpinned).
+ *                  lower model cell); the band against the reference's
+ *                  per-cell acceptance envelope and the range of its draw.
+ *                  The drawn values are parity unpinned: the reference calls
+ *                  rand() inside a parallel loop, so its band is not
+ *                  reproducible; the serial draw order is restated.
+ * The CG solver (cgstep / lin_solver) is restated in numpy in oracle/cg.py.
  */
 #include "born_oracle.h"
 

@@ -7,6 +7,10 @@ domain (source_func_3d.cc:4-74, origin computed once), the wavelet stamp
 map (data_rtm_3d.cc:59-90, map_data_3d.cc:4-35), extract_sub / laplac /
 rtmForward / map_data / deriv for the forward and their counterparts for the
 adjoint.  Single pass over the shots, each shot from a fresh propagator.
+
+exact=True replaces the reference's RTM adjoint (sourceProp + rtmAdjoint)
+by the FP64 transpose of the forward propagation (orc_exact_adjoint_d), the
+operator pair the shipped library runs in its exact mode.
 """
 import ctypes as C
 
@@ -17,8 +21,9 @@ from . import Prop, lib, sinc_table
 
 class OracleBornOp:
     def __init__(self, vn, vo, vd, vel, wavelet, wdt, src_x, src_y, src_depth, nt, t0, dt,
-                 rec_n, rec_o, rec_d, rec_depth, nbound, nthreads=0):
+                 rec_n, rec_o, rec_d, rec_depth, nbound, nthreads=0, exact=False):
         L = lib()
+        self.exact = exact
         self.vn, self.vo, self.vd = list(vn), list(vo), list(vd)
         self.vel = np.ascontiguousarray(vel, np.float32).ravel()
         self.src_x, self.src_y = list(src_x), list(src_y)
@@ -128,11 +133,15 @@ class OracleBornOp:
             L.orc_map_data_adjoint(self.ntd, rnx, rny, self.nt_big, rec, sd)
             rec = rec.reshape(rnx * rny, self.nt_big)
             P.transfer_receiver_func(rl, rec)
-            p0 = np.zeros(n1 * n2 * n3, np.float32)
-            p1 = np.zeros_like(p0)
-            P.source_prop(p0, p1, self.jts, self.nt)
-            img = np.zeros_like(p0)
-            P.rtm_adjoint(p1, p0, img, self.nt)
+            if self.exact:
+                img = P.exact_adjoint_d(self.jtd, rec.astype(np.float64), self.nt)
+                img = np.ascontiguousarray(img, np.float32).ravel()
+            else:
+                p0 = np.zeros(n1 * n2 * n3, np.float32)
+                p1 = np.zeros_like(p0)
+                P.source_prop(p0, p1, self.jts, self.nt)
+                img = np.zeros_like(p0)
+                P.rtm_adjoint(p1, p0, img, self.nt)
             lap = np.zeros_like(img)
             L.orc_laplac_adjoint(n1, n2, n3, lap, img)
             L.orc_add_image(*self.vn, *self.vo, *self.vd, image, n1, n2, n3, *self.do, lap)

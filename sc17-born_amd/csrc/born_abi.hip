@@ -1,6 +1,6 @@
 // C ABI of the Born propagator (include/born_hip.h): context, transfers and
 // the time loops of the reference backend (base/cpu_prop.cc), each step one
-// launch of a fused kernel from born_kernels.hip.
+// launch of a fused kernel of born_ops.h.
 //
 // Exact adjoint.  The Born forward's scattered field obeys
 //   r_{k+1} = A r_k - B^2 r_{k-1} + f_k,   A = B (V L + 2),  d_k = P_k r_{k+1}
@@ -49,6 +49,9 @@ struct born_ctx {
 
   int nsinc_s = 0, jts = 0, nsinc_d = 0, jtd = 0;
   float *d_tab_s = nullptr, *d_tab_d = nullptr;
+  // bumped by every sinc-table transfer: a captured batch time loop holds
+  // the table's device address and is recaptured when this changes
+  long long tab_version = 0;
 
   int npts_s = 0, nt_src = 0;
   float *d_src = nullptr;
@@ -66,6 +69,8 @@ struct born_ctx {
   // wavefield storage of the Born forward / exact adjoint (BORN_STORAGE_*)
   int storage = BORN_STORAGE_F32;
   float vmax2 = 0.f;      // max of velocity 2 (V = v^2 dt^2)
+  float vmin2 = 0.f;      // min of velocity 2 over the stencil interior
+  float bmin = 1.f;       // smallest sponge factor
   float tabd_abs = 0.f;   // largest sum |tab_d[ii][k]| over the rows
   int rec_per_cell = 0;   // most receivers sharing one cell
   float *d_ring = nullptr;    // FP16 level scales, see ring()
@@ -352,6 +357,8 @@ int upload_sponge(born_ctx *ctx) {
   std::vector<float> bound(ctx->nbound);
   const float bcA = (float)ctx->nbound;
   for (int i = 0; i < ctx->nbound; i++) bound[i] = expf(-ctx->bcB * (bcA - i));
+  ctx->bmin = 1.f;
+  for (float b : bound) ctx->bmin = std::min(ctx->bmin, b);
   const int dims[3] = {ctx->nx, ctx->ny, ctx->nz};
   float **dst[3] = {&ctx->d_bx, &ctx->d_by, &ctx->d_bz};
   for (int a = 0; a < 3; a++) {
@@ -709,7 +716,10 @@ int born_create_space(born_ctx *ctx, float d1, float d2, float d3, float bc_a,
   ctx->nbound = 40;
   ctx->bcB = .0005f;
   if (resized) {
+    // every buffer of the old cell count goes, the two-step levels included
+    // (ensure_work / ensure_pair_work only allocate a missing buffer)
     for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &ctx->d_w[i], 0));
+    for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &ctx->d_w2[i], 0));
     CHK(dev_alloc(ctx, &ctx->d_img, 0));
     CHK(dev_alloc(ctx, &ctx->d_acc, 0));
     CHK(dev_alloc(ctx, &ctx->d_p0, 0));
@@ -760,6 +770,7 @@ static int sinc_table(born_ctx *ctx, int nsinc, int jt, const float *table,
   *ns_out = nsinc;
   *jt_out = jt;
   ctx->bg_version++;
+  ctx->tab_version++;
   return BORN_OK;
 }
 
@@ -826,19 +837,28 @@ int born_transfer_vel_func1(born_ctx *ctx, int nx, int ny, int nz, const float *
 
 int born_transfer_vel_func2(born_ctx *ctx, int nx, int ny, int nz, const float *v) {
   if (ctx && v && nx == ctx->nx && ny == ctx->ny && nz == ctx->nz) {
-    // max |v| over thread blocks: the maximum does not depend on the order
-    std::vector<float> part(born_par::threads(), 0.f);
+    // max |v| over the domain and min v over the stencil interior, per
+    // thread block: neither depends on the order
+    std::vector<float> part(born_par::threads(), 0.f), pmin(part.size(), INFINITY);
     const long nb = (long)part.size();
     const long long n = ctx->n;
-    float *pp = part.data();
+    const int nx_ = ctx->nx, ny_ = ctx->ny, nz_ = ctx->nz;
+    float *pp = part.data(), *pm = pmin.data();
     born_par::blocks(nb, 1, [=](long lo, long hi) {
       for (long b = lo; b < hi; b++) {
-        float m = 0.f;
-        for (long long i = n * b / nb; i < n * (b + 1) / nb; i++) m = std::max(m, fabsf(v[i]));
+        float m = 0.f, mn = INFINITY;
+        for (long long i = n * b / nb; i < n * (b + 1) / nb; i++) {
+          m = std::max(m, fabsf(v[i]));
+          const int x = (int)(i % nx_), y = (int)((i / nx_) % ny_), z = (int)(i / ((long long)nx_ * ny_));
+          if (x >= 4 && x < nx_ - 4 && y >= 4 && y < ny_ - 4 && z >= 4 && z < nz_ - 4)
+            mn = std::min(mn, v[i]);
+        }
         pp[b] = m;
+        pm[b] = mn;
       }
     });
     ctx->vmax2 = *std::max_element(part.begin(), part.end());
+    ctx->vmin2 = *std::min_element(pmin.begin(), pmin.end());
   }
   return vel_func(ctx, nx, ny, nz, v, ctx ? &ctx->d_vel2 : nullptr);
 }
@@ -1133,6 +1153,13 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   CHK(check_jt(ctx, jt));
   if (nt < 0 || !traces || !img || (scatter != 0 && scatter != 1))
     return fail(ctx, BORN_ERR_ARG, "bad adjoint arguments");
+  // the adjoint state runs in nu = V B lambda and the image is divided by
+  // V B at the end: V B must be positive in the whole interior
+  if (!(ctx->vmin2 * ctx->bmin > 0.f) || !std::isfinite(ctx->vmax2))
+    return fail(ctx, BORN_ERR_UNSUPPORTED,
+                "the exact adjoint needs velocity 2 and the sponge factors > 0 in the "
+                "interior (min V = " + std::to_string(ctx->vmin2) + ", min sponge = " +
+                    std::to_string(ctx->bmin) + ")");
   CHK(set_device(ctx));
   CHK(ensure_work(ctx));
   CHK(ring_buffers(ctx));
@@ -1626,6 +1653,7 @@ struct born_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int g_jt = -1, g_nt = -1, g_scatter = -1, g_storage = -1, g_shape = -1, g_zc = -1;
+  long long g_tab = -1;  // ctx->tab_version the graph was captured with
 };
 
 namespace {
@@ -1882,7 +1910,7 @@ int born_batch_forward(born_batch *b, int jt, int nt, int scatter) {
   // the time loop: captured once per (jt, nt, scatter, storage, shape), replayed
   const bool same = b->exec && b->g_jt == jt && b->g_nt == nt && b->g_scatter == scatter &&
                     b->g_storage == ctx->storage && b->g_shape == tu.shape &&
-                    b->g_zc == tu.zchunk;
+                    b->g_zc == tu.zchunk && b->g_tab == ctx->tab_version;
   if (!same) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -1909,6 +1937,7 @@ int born_batch_forward(born_batch *b, int jt, int nt, int scatter) {
     b->g_storage = ctx->storage;
     b->g_shape = tu.shape;
     b->g_zc = tu.zchunk;
+    b->g_tab = ctx->tab_version;
   }
   HIPCHK(hipGraphLaunch(b->exec, ctx->stream));
   return BORN_OK;

@@ -224,6 +224,22 @@ __device__ __forceinline__ float leap(float b, float v, float lap, float cur, fl
   return mul_r(b, fma_r(-b, old, add_r(fma_r(v, lap, cur), cur)));
 }
 
+// The Born source term of a forward step and the scatter operand kept for
+// the exact adjoint.  Scatter 0 (image): r += 1e-6 s m in double, the
+// reference's double literal (cpu_prop.cc:220); scatter 1 (dV): sigma =
+// b L(s), r += m sigma.
+template <int SCATTER>
+__device__ __forceinline__ void born_scatter(float bb, float lap_s, float sn, float m,
+                                             float &rn, float &sig) {
+  if (SCATTER == 0) {
+    rn = (float)add_r((double)rn, mul_r(mul_r(.000001, (double)sn), (double)m));
+    sig = sn;
+  } else {
+    sig = mul_r(bb, lap_s);
+    rn = fma_r(m, sig, rn);
+  }
+}
+
 // Eight-tap sinc interpolation with the reference's rounding: products and
 // sums rounded separately, left to right (base/cpu_prop.cc:169-176).
 __device__ __forceinline__ float sinc8(const float *t, const float *v,

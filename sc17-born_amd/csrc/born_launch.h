@@ -1,4 +1,5 @@
-// Host-side launch interface of the fused step kernels (born_kernels.hip).
+// Host-side launch interface of the fused step kernels (ops in born_ops.h, launchers in
+// born_kernels.hip and the born_k_*.hip units).
 #pragma once
 #include <hip/hip_runtime.h>
 

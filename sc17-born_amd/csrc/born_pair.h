@@ -20,7 +20,7 @@
 // Level k+1 outside the interior is zero, as in memory.  A z chunk [z0, z1)
 // computes level k+1 on planes z0-4 .. z1+3 and stores only its own planes.
 //
-// Per cell the arithmetic is that of FwdOp (born_kernels.hip) in the same
+// Per cell the arithmetic is that of FwdOp (born_ops.h) in the same
 // order, so the two steps give the same values as two single-step launches.
 #pragma once
 #include "born_device.h"
@@ -53,22 +53,6 @@ struct PairOp {
   int id_a, id_b, ext_a, ext_b;
 };
 
-// The Born source term of a forward step and the scatter operand kept for
-// the exact adjoint.  Scatter 0 (image): r += 1e-6 s m in double, the
-// reference's double literal (cpu_prop.cc:220); scatter 1 (dV): sigma =
-// b L(s), r += m sigma.
-template <int SCATTER>
-__device__ __forceinline__ void born_scatter(float bb, float lap_s, float sn, float m,
-                                             float &rn, float &sig) {
-  if (SCATTER == 0) {
-    rn = (float)add_r((double)rn, mul_r(mul_r(.000001, (double)sn), (double)m));
-    sig = sn;
-  } else {
-    sig = mul_r(bb, lap_s);
-    rn = fma_r(m, sig, rn);
-  }
-}
-
 // sum of the 25 taps in the order of march_tile: centre, x, y, z pairs
 __device__ __forceinline__ float lap25(const Taps &t, float c, const float (&xs)[2 * kH + 1],
                                        const float (&up)[kH], const float (&dn)[kH],
@@ -83,7 +67,7 @@ __device__ __forceinline__ float lap25(const Taps &t, float c, const float (&xs)
   return a;
 }
 
-// One forward cell update (FwdOp in born_kernels.hip): new s, r and the
+// One forward cell update (FwdOp in born_ops.h): new s, r and the
 // scatter operand (cpu_prop.cc:31-53 for the propagation, :220 for scatter 0).
 template <int SCATTER>
 __device__ __forceinline__ void pair_cell(const PairOp<SCATTER> &op, float bb, float v,

@@ -2,7 +2,7 @@
 //
 // The exact adjoint advances nu = V B lambda backwards with the forward's
 // leapfrog form, injects the receiver residuals and accumulates the image
-// acc += sigma_k nu (LamOp in born_kernels.hip).  One step reads nu (cur,
+// acc += sigma_k nu (LamOp in born_ops.h).  One step reads nu (cur,
 // old), vel, sigma and the image and writes nu and the image: 28 B per
 // cell.  This kernel takes steps k and k-1 in one z sweep with the geometry
 // of pair_kernel (born_pair.h): level j+1 of nu on the tile widened by the

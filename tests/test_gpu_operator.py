@@ -75,3 +75,32 @@ def test_lsrtm_reduces_the_residual(pyborn):
     assert all(r > 0 for r in res)
     assert res[0] < r0 and all(b <= a * (1 + 1e-6) for a, b in zip(res, res[1:]))
     assert res[-1] < 0.5 * r0
+
+
+def _oracle(vel, kw, exact):
+    return OracleBornOp(vel.shape[::-1], kw["vo"], kw["vd"], vel, kw["wavelet"], kw["wdt"],
+                        kw["src_x"], kw["src_y"], kw["src_depth"], kw["nt"], kw["t0"], kw["dt"],
+                        kw["rec_n"], kw["rec_o"], kw["rec_d"], kw["rec_depth"], kw["nbound"],
+                        exact=exact)
+
+
+def test_lsrtm_matches_the_oracle_cg(pyborn):
+    """born_op_lsrtm (simple_solver + cgstep in host/born_op.cc over the HIP
+    exact operator) against the numpy restatement of cgstep.cc:11-54 and
+    lin_solver.cc:104-131 (oracle/cg.py) over the oracle's exact operator:
+    the residual after each of 3 iterations and the final model."""
+    from oracle.cg import simple_solver
+
+    vel, kw, model = _setup(seed=5, nshots=1)
+    O = _oracle(vel, kw, exact=True)
+    data = O.forward(model)
+    H = pyborn.BornOp(vel, mode=1, **kw)
+    m_h, res_h = H.lsrtm(data, 3)
+    H.close()
+    m_o, res_o = simple_solver(O.forward, O.adjoint, data, 3, O.model_shape)
+    print("residuals hip", res_h, "oracle", res_o)
+    assert len(res_o) == 3
+    for a, b in zip(res_h, res_o):
+        assert abs(a - b) <= 1e-4 * b
+    assert res_o[-1] < res_o[0]
+    assert rel_l2(m_h, m_o) <= 1e-4

@@ -408,26 +408,67 @@ def test_rand_subspace_without_band_is_nearest_lower_cell():
     assert O[6, 7, 8] == vv(V[2, 3, 4])            # beyond the model: clamped
 
 
-def test_rand_subspace_band_statistics():
-    """Random band (parity unpinned: the reference draws rand() inside a
-    parallel loop); checked against its acceptance bounds and determinism."""
-    vn = (20, 20, 20)
-    v = np.full(np.prod(vn), 2000.0, np.float32)
-    n = 20
+def _band_dist(n, nr):
+    """vel_fd_3d.cc:57-103: squared normalised distance into the band along
+    one axis, in float as the reference evaluates it"""
+    d2 = np.zeros(n, np.float32)
+    for i in range(n):
+        if i < nr:
+            d2[i] = np.float32((nr - i) * (nr - i)) / np.float32(nr * nr)
+        elif i >= n - nr:
+            k = i - (n - nr - 1)
+            d2[i] = np.float32(k * k) / np.float32(nr * nr)
+    return d2
+
+
+def test_rand_subspace_band_per_cell_envelope():
+    """vel_fd_3d.cc:29-134 with a random band: every cell of the band obeys
+    the reference's acceptance test (:115-122) -- |v + dev| < vmax * 1.03 *
+    (1 - min(1, dist) * .4) and v + dev > 1e-4 -- and the range of its draw,
+    dev = (u - .5 - .49 dist) * 1.3 * dist * v with u in [0, 1]; cells with
+    dist < 1e-4 keep v exactly.  The values of the draws themselves are parity
+    unpinned: the reference calls rand() inside a parallel loop (:50-122)."""
+    vn = (14, 12, 10)
+    z = np.arange(vn[2], dtype=np.float32)
+    v = np.ascontiguousarray(np.broadcast_to((1500 + 120 * z)[:, None, None],
+                                             (vn[2], vn[1], vn[0])), np.float32).ravel()
+    n1, n2, n3 = 20, 22, 18
+    nr = (5, 6, 4)
+    o = (-30.0, -40.0, -20.0)  # domain origin: nearest-lower model cells, clamped
     dt = np.float32(0.001)
-    outs = []
-    for _ in range(2):
-        out = np.zeros(n ** 3, np.float32)
-        L().orc_rand_subspace(42, 6, 6, 6, dt, *vn, 0.0, 0.0, 0.0, 10.0, 10.0, 10.0, v,
-                              n, n, n, 0.0, 0.0, 0.0, 10.0, 10.0, 10.0, out)
-        outs.append(out)
-    assert np.array_equal(outs[0], outs[1])  # serial rand() order is reproducible
-    vel = np.sqrt(outs[0].astype(np.float64)) / float(dt)
-    V = vel.reshape(n, n, n)
-    assert np.allclose(V[6:-6, 6:-6, 6:-6], 2000.0, rtol=1e-6)
-    band = V[:6]
-    assert np.all(band > 0) and np.all(band < 2000 * 1.03 + 1)
-    assert band.std() > 10.0
+    out = np.zeros(n1 * n2 * n3, np.float32)
+    vmax = L().orc_rand_subspace(42, *nr, dt, *vn, 0.0, 0.0, 0.0, 10.0, 10.0, 10.0, v,
+                                 n1, n2, n3, *o, 10.0, 10.0, 10.0, out)
+    assert vmax == v.max()
+    out2 = np.zeros_like(out)
+    L().orc_rand_subspace(42, *nr, dt, *vn, 0.0, 0.0, 0.0, 10.0, 10.0, 10.0, v,
+                          n1, n2, n3, *o, 10.0, 10.0, 10.0, out2)
+    assert np.array_equal(out, out2)  # the serial rand() order is reproducible
+    V = v.reshape(vn[2], vn[1], vn[0])
+    ix = np.clip(((o[0] + 10.0 * np.arange(n1)) / 10.0).astype(int), 0, vn[0] - 1)
+    iy = np.clip(((o[1] + 10.0 * np.arange(n2)) / 10.0).astype(int), 0, vn[1] - 1)
+    iz = np.clip(((o[2] + 10.0 * np.arange(n3)) / 10.0).astype(int), 0, vn[2] - 1)
+    val = V[iz][:, iy][:, :, ix].astype(np.float64)
+    dx2, dy2, dz2 = _band_dist(n1, nr[0]), _band_dist(n2, nr[1]), _band_dist(n3, nr[2])
+    dist = np.sqrt((dx2[None, None, :] + dy2[None, :, None] + dz2[:, None, None])
+                   .astype(np.float32)).astype(np.float64)
+    O = out.reshape(n3, n2, n1)
+    got = np.sqrt(O.astype(np.float64)) / float(dt)  # v + dev
+    inner = dist < 1e-4
+    f32 = np.float32
+    keep = (val.astype(f32) * val.astype(f32) * dt * dt).astype(f32)
+    assert inner.sum() > 0 and np.array_equal(O[inner], keep[inner])
+    band = ~inner
+    lim = float(vmax) * 1.03 * (1.0 - np.minimum(1.0, dist) * 0.4)
+    slack = 1e-5 * got  # float rounding of (v + dev)^2 dt^2 and the square root
+    assert np.all(np.abs(got[band]) < lim[band] + slack[band])
+    assert np.all(got[band] > 1e-4)
+    rel = (got - val) / val
+    lo = 1.3 * dist * (-0.5 - 0.49 * dist)
+    hi = 1.3 * dist * (0.5 - 0.49 * dist)
+    assert np.all(rel[band] >= lo[band] - 1e-5) and np.all(rel[band] <= hi[band] + 1e-5)
+    # the band is drawn, not copied: most cells move
+    assert np.mean(np.abs(rel[band]) > 1e-6) > 0.95
 
 
 # ------------------------------------------------------------ whole loops
