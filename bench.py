@@ -28,24 +28,28 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "sc17-born_amd")
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes per interior cell and step, FP32 / FP16 levels
-# (vel, model, image and traces stay FP32):
+# Algorithmic HBM bytes per interior cell and step, FP32 / FP16 / 24-bit
+# levels (4 / 2 / 3 bytes per cell; vel, model, image and traces stay FP32):
 #   forward: s, r read cur + old, vel, model; s, r written (FP32: s into
-#     the kept history with scatter 0; FP16: s and r as one __half2 per cell
-#     and the history written apart)                            32 / 22
+#     the kept history with scatter 0; FP16 and 24-bit: s and r stored
+#     together per cell and the history written apart)         32 / 22 / 29
 #   adjoint state: nu read cur + old, vel, sigma, image; nu and image
-#     written                                                   28 / 20
+#     written                                                   28 / 20 / 24
 #   background recompute (steps of checkpoint segments), scatter 0: s read
-#     cur + old, vel; s written (into the segment history)      16 / 10
+#     cur + old, vel; s written (into the segment history)      16 / 10 / 13
+#   background sweep of an adjoint whose forward kept no history (the same
+#     kernel over every step, storing checkpoints)              16 / 10 / 13
 FWD_BYTES_PER_CELL = 32
-BYTES = {"f32": {"fwd": 32, "lam": 28, "srec": 16}, "f16": {"fwd": 22, "lam": 20, "srec": 10}}
+BYTES = {"f32": {"fwd": 32, "lam": 28, "srec": 16}, "f16": {"fwd": 22, "lam": 20, "srec": 10},
+         "f24": {"fwd": 29, "lam": 24, "srec": 13}}
 
 
-def born_bytes(storage, nt, recomputed):
+def born_bytes(storage, nt, recomputed, swept=0):
     """Algorithmic bytes per Born cell-step of one forward + exact adjoint,
-    for the checkpoint plan that ran (recomputed background steps)."""
+    for the checkpoint plan that ran (recomputed background steps, and the
+    background steps swept again when the forward kept no history)."""
     b = BYTES[storage]
-    return b["fwd"] + b["lam"] + b["srec"] * recomputed / nt
+    return b["fwd"] + b["lam"] + b["srec"] * (recomputed + swept) / nt
 METRIC = "Born fwd+adj Mcell-updates/s + HBM GB/s vs peak; FP16-store batch vs FP32 plain"
 # HBM bytes per launch of the forward step kernel on the 304^3 workload, from
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of this script
@@ -206,8 +210,8 @@ def batch_leg(args, pyborn, configs, torch):
                           "src_locs": s.src_locs, "src_vals": s.src_vals, "rec_nx": s.rec_nx,
                           "rec_ny": s.rec_ny, "rec_locs": s.rec_locs} for s in shots],
                      (10.0, 10.0, 10.0))
-    out = {}
-    for storage in ("f32", "f16"):
+    out, err, tile = {}, {}, {}
+    for storage in ("f32", "f16", "f24"):
         P.set_storage(storage)
         B.forward(1, nt, 0)  # autotune and graph capture
         P.synchronize()
@@ -216,12 +220,13 @@ def batch_leg(args, pyborn, configs, torch):
         P.synchronize()
         out[storage] = time.perf_counter() - t0
         log("batch leg: %s batch %.2f s" % (storage, out[storage]))
-    d16 = B.traces(0)
-    err = float(np.linalg.norm(d16 - ref0) / np.linalg.norm(ref0))
-    tile = B.tile(0)
+        d = B.traces(0)
+        err[storage] = float(np.linalg.norm(d - ref0) / np.linalg.norm(ref0))
+        tile[storage] = B.tile(0)
     B.close()
     P.close()
     mcu = lambda t: round(2 * cells * nt / t / 1e6, 1)
+    pair = None if args.no_batch_pair else pair_leg(args, pyborn, shots, tab, nt, torch)
     return {
         "workload": "Born forward, %d shots on windows of [%d, %d]^2 x %d cells at 10 m "
                     "(padded per shot), 96x96 receivers each, %d steps"
@@ -232,9 +237,103 @@ def batch_leg(args, pyborn, configs, torch):
         "f32_batch": mcu(out["f32"]),
         "f16_batch": mcu(out["f16"]),
         "f16_batch_speedup_vs_f32_plain": round(t_single / out["f16"], 3),
-        "f16_batch_tile": tile,
-        "f16_vs_f32_data_rel_l2_shot0": round(err, 6),
+        "f16_batch_tile": tile["f16"],
+        "f16_vs_f32_data_rel_l2_shot0": round(err["f16"], 6),
+        "f24_batch": mcu(out["f24"]),
+        "f24_batch_speedup_vs_f32_plain": round(t_single / out["f24"], 3),
+        "f24_batch_tile": tile["f24"],
+        "f24_vs_f32_data_rel_l2_shot0": round(err["f24"], 7),
+        "pair": pair,
     }
+
+
+def pair_leg(args, pyborn, shots, tab, nt, torch):
+    """The Born pair (forward, then the exact adjoint of its traces) on the
+    first args.pair_shots windowed shots: FP32 one shot at a time on its own
+    context (the background history kept by the forward when it fits), against
+    one FP16 and one 24-bit batch running born_batch_forward then
+    born_batch_adjoint_exact (background rebuilt from checkpoints of the whole
+    batch).  Per-shot image errors against the FP32 images."""
+    sel = shots[:args.pair_shots]
+    cells = sum(s.interior_cells for s in sel)
+    dev = torch.device("cuda:0")
+
+    def load(P, s):
+        P.create_space(10.0, 10.0, 10.0, s.nx, s.ny, s.nz)
+        P.transfer_sinc_table_s(tab)
+        P.transfer_sinc_table_d(tab)
+        P.transfer_vel_func1(s.V)
+        P.transfer_vel_func2(s.V)
+        P.transfer_receiver_func(s.rec_nx, s.rec_ny, s.rec_locs,
+                                 np.zeros((s.nrec, s.nt_big), np.float32))
+        P.transfer_source_func(s.src_locs, s.src_vals)
+
+    # FP32 single-shot pairs; the launch shapes tuned on the first shot are
+    # fixed for the others (every shot has the same nz), so no tuning is timed
+    tuned, ref, t32 = None, [], 0.0
+    for k, s in enumerate(sel):
+        P = pyborn.Propagator(0)
+        load(P, s)
+        m = torch.from_numpy(s.model).to(dev)
+        tr = torch.zeros((s.nt_big, s.nrec), dtype=torch.float32, device=dev)
+        img = torch.zeros(s.nx * s.ny * s.nz, dtype=torch.float32, device=dev)
+        if tuned is None:
+            P.dev_forward(1, nt, m, tr, scatter=0, keep_background=True)
+            P.dev_adjoint_exact(1, nt, tr, img, scatter=0)
+            P.synchronize()
+            tuned = {k2: v for k2, v in P.tuning().items()
+                     if k2 in ("fwd", "background", "adjoint_state") and v["ms"] >= 0}
+            tr.zero_()
+            img.zero_()
+        else:
+            for name, t in tuned.items():
+                P.set_tuning(name, t["rows_per_thread"], t["waves"], t["cells_per_lane"],
+                             t["lookahead"], t["zchunk"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        P.dev_forward(1, nt, m, tr, scatter=0, keep_background=True)
+        P.dev_adjoint_exact(1, nt, tr, img, scatter=0)
+        P.synchronize()
+        t32 += time.perf_counter() - t0
+        ref.append(img.cpu().numpy())
+        P.close()
+        del m, tr, img
+        torch.cuda.empty_cache()
+    log("pair leg: FP32 one shot at a time %.2f s" % t32)
+    out = {"workload": "Born forward + exact adjoint of the first %d of the windowed shots, "
+                       "%d steps" % (len(sel), nt),
+           "interior_cells": cells,
+           "unit": "Mcell-updates/s (4 wavefield updates per Born cell-step)",
+           "f32_one_shot_at_a_time": round(4 * cells * nt / t32 / 1e6, 1)}
+    P = pyborn.Propagator(0)
+    P.transfer_sinc_table_s(tab)
+    P.transfer_sinc_table_d(tab)
+    B = pyborn.Batch(P, [{"nx": s.nx, "ny": s.ny, "nz": s.nz, "vel": s.V, "model": s.model,
+                          "src_locs": s.src_locs, "src_vals": s.src_vals, "rec_nx": s.rec_nx,
+                          "rec_ny": s.rec_ny, "rec_locs": s.rec_locs} for s in sel],
+                     (10.0, 10.0, 10.0))
+    for storage in ("f16", "f24"):
+        P.set_storage(storage)
+        B.forward(1, nt, 0)  # autotune and graph capture
+        B.adjoint_exact(1, nt, 0)
+        P.synchronize()
+        t0 = time.perf_counter()
+        B.forward(1, nt, 0)
+        B.adjoint_exact(1, nt, 0)
+        P.synchronize()
+        t = time.perf_counter() - t0
+        segs, rec = B.last_checkpoint_plan()
+        errs = [float(np.linalg.norm(B.image(k) - r) / np.linalg.norm(r)) for k, r in enumerate(ref)]
+        log("pair leg: %s batch %.2f s, %d segments" % (storage, t, segs))
+        out[storage + "_batch"] = round(4 * cells * nt / t / 1e6, 1)
+        out[storage + "_batch_speedup_vs_f32_plain"] = round(t32 / t, 3)
+        out[storage + "_checkpoint_segments"] = segs
+        out[storage + "_recomputed_steps"] = rec
+        out[storage + "_image_rel_l2_vs_f32_max"] = round(max(errs), 7)
+        out[storage + "_image_rel_l2_vs_f32_per_shot"] = [round(e, 7) for e in errs]
+    B.close()
+    P.close()
+    return out
 
 
 def main():
@@ -249,10 +348,19 @@ def main():
                     help="HBM budget for the background history (0: free memory)")
     ap.add_argument("--nt", type=int, default=0, help="time steps (profiling runs)")
     ap.add_argument("--shots", type=int, default=0, help="shots (profiling runs)")
-    ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage leg")
+    ap.add_argument("--no-f16", action="store_true",
+                    help="skip the FP16- and 24-bit-storage legs")
+    ap.add_argument("--no-checkpointed", action="store_true",
+                    help="skip the leg whose HBM budget forces a checkpointed background")
+    ap.add_argument("--leg-steps", type=int, default=3,
+                    help="timed steps of the legs after the headline one (at most --steps)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-shots leg")
     ap.add_argument("--batch-nt", type=int, default=0, help="time steps of the batch leg")
     ap.add_argument("--batch-shots", type=int, default=0, help="shots of the batch leg")
+    ap.add_argument("--no-batch-pair", action="store_true",
+                    help="skip the Born pair on the first shots of the batch")
+    ap.add_argument("--pair-shots", type=int, default=8,
+                    help="shots of the batch's Born pair sub-leg")
     ap.add_argument("--fwd-shape", default="",
                     help="fix the FP32 forward launch shape: rows,waves,cells_per_lane"
                          "[,lookahead] (64-cell rows; profiling runs), e.g. 1,8,1 or 2,8,1,2")
@@ -314,10 +422,12 @@ def main():
     stream = torch.cuda.ExternalStream(P.stream)
     nt = p.nt
 
-    def run_leg(storage):
-        """warmup + timed steps with the given wavefield storage; returns
-        (elapsed s, forward events, adjoint events)"""
+    def run_leg(storage, steps, warmup, budget=0):
+        """warmup + timed steps with the given wavefield storage and history
+        budget (0: free memory); returns (elapsed s, forward events,
+        adjoint events)"""
         P.set_storage(storage)
+        P.set_checkpoint_budget(budget)
         fwd_events, adj_events = [], []
 
         def step(record):
@@ -338,27 +448,42 @@ def main():
                     adj_events.append((e[1], e[2]))
             P.synchronize()
 
-        for i in range(args.warmup):
+        for i in range(warmup):
             step(False)
             log("%s warmup step %d done" % (storage, i))
         torch.cuda.synchronize()
         P.synchronize()
         t0 = time.perf_counter()
-        for i in range(args.steps):
+        for i in range(steps):
             step(True)
             log("%s timed step %d done" % (storage, i))
         torch.cuda.synchronize()
         P.synchronize()
         return time.perf_counter() - t0, fwd_events, adj_events
 
-    elapsed, fwd_events, adj_events = run_leg("f32")
+    budget = int(args.budget_gb * 2 ** 30)
+    elapsed, fwd_events, adj_events = run_leg("f32", args.steps, args.warmup, budget)
     segs, recomputed = P.last_checkpoint_plan()
+    # the history fits the budget: the forward kept it and the adjoint swept
+    # no background; else the adjoint swept all steps before recomputing
+    swept = 0 if segs == 1 and recomputed == 0 and budget == 0 else nt
     tuning = P.tuning()
-    f16 = None
+    leg_steps = max(1, min(args.steps, args.leg_steps))
+    leg_warm = min(args.warmup, 1)
+    legs = {}
     if not args.no_f16:
-        el16, fe16, ae16 = run_leg("f16")
-        f16 = (el16, fe16, ae16, P.last_checkpoint_plan())
+        for storage in ("f16", "f24"):
+            el, fe, ae = run_leg(storage, leg_steps, leg_warm)
+            legs[storage] = (el, fe, ae, P.last_checkpoint_plan())
         tuning = P.tuning()
+    ckpt = None
+    if not args.no_checkpointed and not args.budget_gb:
+        # a budget of half the FP32 history: two segments, the second half of
+        # the background recomputed once from a checkpoint
+        half_gb = nt * p.nx * p.ny * p.nz * 4 / 2 ** 30 / 2 + 1
+        elc, fec, aec = run_leg("f32", leg_steps, leg_warm, int(half_gb * 2 ** 30))
+        ckpt = (elc, fec, aec, P.last_checkpoint_plan(), half_gb)
+        P.set_checkpoint_budget(0)
 
     nshots = len(p.shots)
     cell_steps = p.interior_cells * nt * nshots * args.steps
@@ -369,7 +494,7 @@ def main():
     fwd_launch_s = fwd_ms / 1e3 / nt
     fwd_alg_bytes = FWD_BYTES_PER_CELL * p.interior_cells
     achieved = fwd_alg_bytes / fwd_launch_s / 1e9
-    alg_gbs = born_bytes("f32", nt, recomputed) * cell_steps / elapsed / 1e9
+    alg_gbs = born_bytes("f32", nt, recomputed, swept) * cell_steps / elapsed / 1e9
     ft = tuning["fwd"]
     fwd_kernel = "FwdOp<0, float, true>, %d, %d, %d, %d, %d" % (
         ft["rows_per_thread"], ft["waves"], ft["cells_per_lane"], ft["lookahead"],
@@ -397,7 +522,13 @@ def main():
             "global_batch": nshots,
             "parallelism": "single GPU",
             "born_cell_steps_per_s": round(cell_steps / elapsed, 1),
-            "bytes_per_born_cell_step": round(born_bytes("f32", nt, recomputed), 2),
+            "bytes_per_born_cell_step": round(born_bytes("f32", nt, recomputed, swept), 2),
+            # the unit counts 4 wavefield updates per Born cell-step (s and r
+            # in the forward, background and adjoint state in the adjoint);
+            # with the background history kept by the forward the adjoint
+            # executes only the adjoint-state update, 3 of the 4 counted
+            "updates_executed_per_born_cell_step": 3 if swept == 0 and recomputed == 0 else
+                                                   round(3 + (recomputed + swept) / nt, 3),
             "algorithmic_GBps": round(alg_gbs, 1),
             "algorithmic_frac_of_hbm": round(alg_gbs / HBM_PEAK_GBS, 4),
             "forward_ms_per_shot": round(float(fwd_ms), 2),
@@ -418,20 +549,45 @@ def main():
             "avg_launch_us": round(fwd_launch_s * 1e6, 2),
         },
     }
-    if f16 is not None:
-        el16, fe16, ae16, (segs16, rec16) = f16
-        v16 = 4 * cell_steps / el16 / 1e6
-        out["config"]["f16_storage"] = {
-            "value": round(v16, 1),
+    leg_cells = p.interior_cells * nt * nshots * leg_steps
+    names = {"f16": "f16_storage", "f24": "f24_storage"}
+    for storage, (el, fe, ae, (sg, rc)) in legs.items():
+        v = 4 * leg_cells / el / 1e6
+        out["config"][names[storage]] = {
+            "value": round(v, 1),
             "unit": "Mcell-updates/s",
-            "ms_per_step": round(el16 / args.steps * 1e3, 2),
-            "speedup_vs_f32": round(v16 / value, 3),
-            "forward_ms_per_shot": round(float(np.mean([a.elapsed_time(b) for a, b in fe16])), 2),
-            "adjoint_ms_per_shot": round(float(np.mean([a.elapsed_time(b) for a, b in ae16])), 2),
-            "checkpoint_segments": segs16,
-            "recomputed_steps": rec16,
-            "bytes_per_born_cell_step": round(born_bytes("f16", nt, rec16), 2),
-            "algorithmic_GBps": round(born_bytes("f16", nt, rec16) * cell_steps / el16 / 1e9, 1),
+            "timed_steps": leg_steps,
+            "ms_per_step": round(el / leg_steps * 1e3, 2),
+            "speedup_vs_f32": round(v / value, 3),
+            "forward_ms_per_shot": round(float(np.mean([a.elapsed_time(b) for a, b in fe])), 2),
+            "adjoint_ms_per_shot": round(float(np.mean([a.elapsed_time(b) for a, b in ae])), 2),
+            "checkpoint_segments": sg,
+            "recomputed_steps": rc,
+            "bytes_per_born_cell_step": round(born_bytes(storage, nt, rc), 2),
+            "algorithmic_GBps": round(born_bytes(storage, nt, rc) * leg_cells / el / 1e9, 1),
+        }
+    if ckpt is not None:
+        elc, fec, aec, (sgc, rcc), gb = ckpt
+        vc = 4 * leg_cells / elc / 1e6
+        bc = born_bytes("f32", nt, rcc, nt)
+        bg = tuning.get("background", {})
+        out["config"]["checkpointed"] = {
+            "workload": "the same pair with a %.0f GiB history budget: the forward keeps no "
+                        "history, the adjoint sweeps the background once storing checkpoints "
+                        "and recomputes earlier segments" % gb,
+            "value": round(vc, 1),
+            "unit": "Mcell-updates/s",
+            "timed_steps": leg_steps,
+            "ms_per_step": round(elc / leg_steps * 1e3, 2),
+            "forward_ms_per_shot": round(float(np.mean([a.elapsed_time(b) for a, b in fec])), 2),
+            "adjoint_ms_per_shot": round(float(np.mean([a.elapsed_time(b) for a, b in aec])), 2),
+            "checkpoint_segments": sgc,
+            "recomputed_steps": rcc,
+            "background_steps_per_adjoint": nt + rcc,
+            "updates_executed_per_born_cell_step": round(3 + (nt + rcc) / nt, 3),
+            "bytes_per_born_cell_step": round(bc, 2),
+            "algorithmic_GBps": round(bc * leg_cells / elc / 1e9, 1),
+            "background_kernel_tuned_us": round(bg.get("ms", -1) * 1e3, 1),
         }
     P.close()
     del model, traces, img

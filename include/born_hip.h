@@ -159,6 +159,13 @@ int born_last_checkpoint_plan(const born_ctx *ctx, int *segments,
  * FP32. */
 #define BORN_STORAGE_F32 0
 #define BORN_STORAGE_F16 1
+/* BORN_STORAGE_F24: every level as an FP16 value plus a refinement byte per
+ * cell (3 bytes), with the same per-level scales as BORN_STORAGE_F16: a level
+ * is stored to 2^20 steps of its range instead of FP16's 11 significant bits,
+ * so the rounding noise injected at every step no longer dominates the error
+ * of thousands of steps.  Kernels take one cell per lane; fields of 24-bit
+ * levels hold fewer than 715,827,882 cells. */
+#define BORN_STORAGE_F24 2
 int born_set_storage(born_ctx *ctx, int storage);
 
 /* Two time steps per launch with FP32 storage, in the Born forward and in
@@ -183,7 +190,8 @@ int born_set_autotune(born_ctx *ctx, int on);
 /* The chosen launch shape of a step kernel: 0 Born forward, 1 Born forward
  * (dV scatter), 2 source propagation, 3 reference adjoint, 4/5 background
  * recompute, 6 adjoint state; 7-11 the FP16-storage forms of 0, 1, 4, 5, 6;
- * 12, 13, 14 the two-step forms of 0, 1, 6 (tile 56 x rows*waves).
+ * 12, 13, 14 the two-step forms of 0, 1, 6 (tile 56 x rows*waves); 15-19
+ * the 24-bit-storage forms of 0, 1, 4, 5, 6.
  * ms = timed duration of one launch (-1 when not tuned). */
 int born_get_tuning(const born_ctx *ctx, int kernel, int *rows_per_thread,
                     int *waves_per_block, int *zchunk, float *ms);
@@ -233,6 +241,27 @@ int born_batch_forward(born_batch *b, int jt, int nt, int scatter);
 /* Traces of one shot after born_batch_forward, [nrec][nt_big] host memory
  * (synchronises the stream). */
 int born_batch_traces(born_batch *b, int shot, float *rec);
+/* Exact adjoint of born_batch_forward for every shot of the batch (the
+ * transpose of born_rtm_forward per shot, scatter 0, or of the dV scatter,
+ * born_adjoint_exact per shot): image_s = F_s^T traces_s, into the batch's
+ * per-shot images (born_batch_image).  The traces are those of the last
+ * born_batch_forward, or those set with born_batch_set_traces.  One launch
+ * per time step covers every shot; the background is rebuilt from HBM
+ * checkpoints of the whole batch, the fewest segments that fit the budget
+ * (born_set_checkpoint_budget on ctx; 0: free memory).  Queued on the
+ * context stream.  Each shot's image equals born_adjoint_exact of that shot
+ * alone.  The reference migrates one shot per process and sums the images
+ * (base/rtm_zero_op_3d.cc:240-299, util/Add.cc); there is no batched
+ * counterpart. */
+int born_batch_adjoint_exact(born_batch *b, int jt, int nt, int scatter);
+/* host traces [nrec][nt_big] of one shot as the adjoint's input */
+int born_batch_set_traces(born_batch *b, int shot, const float *rec);
+/* image of one shot after born_batch_adjoint_exact: host [nz][ny][nx] of the
+ * shot's domain (synchronises the stream) */
+int born_batch_image(born_batch *b, int shot, float *img);
+/* segments and recomputed steps of the last born_batch_adjoint_exact */
+int born_batch_last_checkpoint_plan(const born_batch *b, int *segments,
+                                    int *recomputed_steps);
 /* Tile (cells) and z chunk the autotuner chose for the batch kernel. */
 int born_batch_get_tile(const born_batch *b, int scatter, int *tile_x, int *tile_y,
                         int *zchunk, float *ms);

@@ -155,6 +155,12 @@ def sinc_table(jt, nsinc=8):
     return out
 
 
+def default_threads():
+    """OMP_NUM_THREADS when set, else the CPUs this process may run on"""
+    env = os.environ.get("OMP_NUM_THREADS")
+    return int(env) if env else len(os.sched_getaffinity(0))
+
+
 def f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
@@ -167,7 +173,7 @@ class Prop:
 
     def __init__(self, nthreads=0, variant=None):
         self.s = OrcProp()
-        self.s.nthreads = nthreads or (os.cpu_count() or 1)
+        self.s.nthreads = nthreads or default_threads()
         self._keep = {}
         self.L = lib() if variant is None else variant_lib(variant)
 

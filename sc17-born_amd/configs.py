@@ -231,6 +231,33 @@ CONFIGS = {
                rec_o=10.0, seed=1002),
 }
 
+# Operator-level form of the 256^3 pair (born_op_* / pyborn.BornOp): the
+# velocity model and the rtm_zero_op parameters of the C2 problem above (the
+# shot positions are make_problem's draws for the same seed)
+def op_config(name="c2", mode=1, scatter=0, storage="f32"):
+    cfg = CONFIGS[name]
+    mx, my, mz = cfg["n"]
+    d = 10.0
+    vz = velocity_law(mz, d)
+    vel = np.ascontiguousarray(np.broadcast_to(vz[:, None, None], (mz, my, mx)), np.float32)
+    rng = np.random.default_rng(cfg["seed"])
+    sx, sy = [], []
+    for k in range(cfg["nshots"]):
+        if "src_xy" in cfg:
+            x, y = cfg["src_xy"][k]
+        else:
+            x = float(rng.uniform(0.25, 0.75) * (mx - 1) * d)
+            y = float(rng.uniform(0.25, 0.75) * (my - 1) * d)
+        sx.append(x)
+        sy.append(y)
+    kw = dict(vo=(0.0, 0.0, 0.0), vd=(d, d, d), wavelet=ricker(200, 0.001, 12.0, 0.1),
+              wdt=0.001, src_x=np.array(sx, np.float32), src_y=np.array(sy, np.float32),
+              src_depth=20.0, nt=cfg["nt"], t0=0.0, dt=0.001, rec_n=cfg["rec"],
+              rec_o=(cfg["rec_o"], cfg["rec_o"]), rec_d=(cfg["rec_d"], cfg["rec_d"]),
+              rec_depth=10.0, nbound=20, mode=mode, scatter=scatter, storage=storage)
+    return vel, kw
+
+
 # FP16 error study: the 512x512x400 grid's velocity (v(z) plus 4 seeded
 # Gaussian anomalies of +-300 m/s, radius 40 cells), 2500 steps at 0.8 ms,
 # receivers every 20 m, restricted to a 128^3 crop around the shot (the FP64

@@ -43,8 +43,8 @@ struct born_ctx {
   float *d_bx = nullptr, *d_by = nullptr, *d_bz = nullptr;
   int zchunk = 0;  // override of the tuned z chunk (0: tuned)
   int autotune = 1;
-  born::Tune tune[16];
-  float tune_ms[16];
+  born::Tune tune[24];
+  float tune_ms[24];
   int pairs = 0;  // two-step forward launches for FP32 levels: 0 off, 1 auto, 2 on
 
   int nsinc_s = 0, jts = 0, nsinc_d = 0, jtd = 0;
@@ -80,6 +80,7 @@ struct born_ctx {
   size_t ckring_cap = 0;
 
   float *d_w[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t w_cap = 0;  // floats per work buffer
   float *d_w2[4] = {nullptr, nullptr, nullptr, nullptr};  // levels k+1, k+2 of pairs
   float *d_img = nullptr, *d_acc = nullptr, *d_p0 = nullptr, *d_p1 = nullptr;
   float *d_tr = nullptr, *d_tr2 = nullptr;
@@ -187,9 +188,22 @@ born::RecMap recmap(const born_ctx *c) {
 // step kernels that are tuned separately
 enum Kind {
   K_FWD0, K_FWD1, K_SRC, K_RADJ, K_SREC0, K_SREC1, K_LAM,
-  K_FWD0H, K_FWD1H, K_SREC0H, K_SREC1H, K_LAMH, K_FWD0P, K_FWD1P, K_LAMP, K_NUM
+  K_FWD0H, K_FWD1H, K_SREC0H, K_SREC1H, K_LAMH, K_FWD0P, K_FWD1P, K_LAMP,
+  K_FWD0S, K_FWD1S, K_SREC0S, K_SREC1S, K_LAMS, K_NUM
 };
-static_assert(K_NUM <= 16, "tuning slots");
+static_assert(K_NUM <= 24, "tuning slots");
+// the step kernels of a storage form (BORN_STORAGE_*)
+int fwd_kind(int storage, int scatter) {
+  static const int k[3][2] = {{K_FWD0, K_FWD1}, {K_FWD0H, K_FWD1H}, {K_FWD0S, K_FWD1S}};
+  return k[storage][scatter];
+}
+int srec_kind(int storage, int scatter) {
+  static const int k[3][2] = {{K_SREC0, K_SREC1}, {K_SREC0H, K_SREC1H}, {K_SREC0S, K_SREC1S}};
+  return k[storage][scatter];
+}
+int lam_kind(int storage) { return storage == 2 ? K_LAMS : storage == 1 ? K_LAMH : K_LAM; }
+// kernels of the 24-bit storage take one cell per lane
+bool one_cell(int kind) { return kind >= K_FWD0S && kind <= K_LAMS; }
 bool is_pair(int kind) { return kind == K_FWD0P || kind == K_FWD1P || kind == K_LAMP; }
 
 born::SrcPts points(const born_ctx *c) { return {c->d_pts, c->npts_s}; }
@@ -296,8 +310,8 @@ int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx,
 
 template <class F>
 int autotune(born_ctx *ctx, int kind, F run) {
-  return autotune_into(ctx, &ctx->tune[kind], &ctx->tune_ms[kind], ctx->nx % 2 == 0,
-                       ctx->nz - 2 * born::kH, run);
+  return autotune_into(ctx, &ctx->tune[kind], &ctx->tune_ms[kind],
+                       ctx->nx % 2 == 0 && !one_cell(kind), ctx->nz - 2 * born::kH, run);
 }
 
 // The same for a two-step kernel: its shapes and z chunks that split the
@@ -395,9 +409,28 @@ int check_dims(born_ctx *ctx, int n1, int n2, int n3) {
   return BORN_OK;
 }
 
+// floats per work buffer: one float level, or a 24-bit level of two fields
+// stored together (6 bytes per cell)
+size_t work_floats(const born_ctx *c) {
+  return c->storage == BORN_STORAGE_F24 ? ((size_t)c->n * 3 + 1) / 2 : (size_t)c->n;
+}
+
 int ensure_work(born_ctx *ctx) {
-  for (int i = 0; i < 4; i++)
-    if (!ctx->d_w[i]) CHK(dev_alloc(ctx, &ctx->d_w[i], (size_t)ctx->n));
+  const size_t need = work_floats(ctx);
+  if (ctx->w_cap < need) {
+    ctx->w_cap = 0;
+    for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &ctx->d_w[i], need));
+    ctx->w_cap = need;
+  }
+  return BORN_OK;
+}
+
+// 24-bit levels of two fields are addressed with 32-bit byte offsets up to
+// 6 bytes per cell
+int check_storage(born_ctx *ctx) {
+  if (ctx->storage == BORN_STORAGE_F24 && (unsigned long long)ctx->n * 6 >= (1ull << 32))
+    return fail(ctx, BORN_ERR_UNSUPPORTED,
+                "24-bit storage needs fewer than 715,827,882 cells (32-bit byte offsets)");
   return BORN_OK;
 }
 
@@ -505,7 +538,10 @@ int traces_out(born_ctx *ctx, float *h) {
   return BORN_OK;
 }
 
-size_t elem(const born_ctx *c) { return c->storage == BORN_STORAGE_F16 ? 2 : 4; }
+// bytes per cell of one stored level
+size_t elem(const born_ctx *c) {
+  return c->storage == BORN_STORAGE_F16 ? 2 : c->storage == BORN_STORAGE_F24 ? 3 : 4;
+}
 
 // k-th field of element size esz in an array of fields of n cells
 float *nth(float *base, long long k, long long n, size_t esz) {
@@ -551,7 +587,7 @@ float tap_sum(const born_ctx *c) {
 
 born::Half half_state(const born_ctx *c, int k) {
   born::Half h;
-  h.on = c->storage == BORN_STORAGE_F16;
+  h.on = c->storage;  // BORN_STORAGE_F32 / F16 / F24 = 0 / 1 / 2
   h.sl = slots(k);
   h.tapsum = tap_sum(c);
   h.gain = 2.f + c->vmax2 * h.tapsum;
@@ -719,6 +755,7 @@ int born_create_space(born_ctx *ctx, float d1, float d2, float d3, float bc_a,
     // every buffer of the old cell count goes, the two-step levels included
     // (ensure_work / ensure_pair_work only allocate a missing buffer)
     for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &ctx->d_w[i], 0));
+    ctx->w_cap = 0;
     for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &ctx->d_w2[i], 0));
     CHK(dev_alloc(ctx, &ctx->d_img, 0));
     CHK(dev_alloc(ctx, &ctx->d_acc, 0));
@@ -901,10 +938,11 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   if (nt < 0 || !model || !traces || (scatter != 0 && scatter != 1))
     return fail(ctx, BORN_ERR_ARG, "bad forward arguments");
   CHK(set_device(ctx));
+  CHK(check_storage(ctx));
   CHK(ensure_work(ctx));
   CHK(ring_buffers(ctx));
   const long long n = ctx->n;
-  const bool half = ctx->storage == BORN_STORAGE_F16;
+  const bool half = ctx->storage != BORN_STORAGE_F32;  // scaled 16-bit levels
   const size_t esz = elem(ctx);
   const bool try_pairs = ctx->pairs && !half && nt >= 2;
   if (try_pairs) CHK(ensure_pair_work(ctx));
@@ -919,7 +957,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   ctx->hist_version = -1;
   ctx->jt_state = jt;
   const Geometry geo = geometry(ctx);
-  const int kind = half ? (scatter == 0 ? K_FWD0H : K_FWD1H) : (scatter == 0 ? K_FWD0 : K_FWD1);
+  const int kind = fwd_kind(ctx->storage, scatter);
   {  // tune on the work fields (zeroed below) and scratch scales, no traces written
     CHK(ring_reset(ctx, RING_T0));
     CHK(ring_reset(ctx, RING_T1));
@@ -965,7 +1003,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w2[i], n));
   }
   ctx->last_pairs = use_pairs;
-  for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w[i], n));
+  for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w[i], (long long)ctx->w_cap));
   CHK(source_amps(ctx, nt, jt, 0, 1 / (float)jt));
   if (half) {
     CHK(ring_reset(ctx, RING_S));
@@ -1161,19 +1199,19 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
                 "interior (min V = " + std::to_string(ctx->vmin2) + ", min sponge = " +
                     std::to_string(ctx->bmin) + ")");
   CHK(set_device(ctx));
+  CHK(check_storage(ctx));
   CHK(ensure_work(ctx));
   CHK(ring_buffers(ctx));
   CHK(field_buf(ctx, &ctx->d_acc));
   const long long n = ctx->n;
   if (nt == 0) return BORN_OK;
-  const bool half = ctx->storage == BORN_STORAGE_F16;
+  const bool half = ctx->storage != BORN_STORAGE_F32;  // scaled 16-bit levels
   const bool try_pairs = ctx->pairs && !half && nt >= 2;
   if (try_pairs) CHK(ensure_pair_work(ctx));
   const size_t esz = elem(ctx);
   const Geometry geo = geometry(ctx);
-  const int skind = half ? (scatter == 0 ? K_SREC0H : K_SREC1H)
-                         : (scatter == 0 ? K_SREC0 : K_SREC1);
-  const int lkind = half ? K_LAMH : K_LAM;
+  const int skind = srec_kind(ctx->storage, scatter);
+  const int lkind = lam_kind(ctx->storage);
   // bound of |V B P^T w| per unit of max |trace| (FP16 scales)
   const float inj = ctx->vmax2 * ctx->tabd_abs * (float)ctx->rec_per_cell;
   {  // tune both kernels on work fields, the accumulator and scratch scales
@@ -1537,8 +1575,10 @@ int born_last_step_pairs(const born_ctx *ctx) {
 
 int born_set_storage(born_ctx *ctx, int storage) {
   if (!ctx) return BORN_ERR_ARG;
-  if (storage != BORN_STORAGE_F32 && storage != BORN_STORAGE_F16)
-    return fail(ctx, BORN_ERR_ARG, "storage must be BORN_STORAGE_F32 or BORN_STORAGE_F16");
+  if (storage != BORN_STORAGE_F32 && storage != BORN_STORAGE_F16 &&
+      storage != BORN_STORAGE_F24)
+    return fail(ctx, BORN_ERR_ARG,
+                "storage must be BORN_STORAGE_F32, BORN_STORAGE_F16 or BORN_STORAGE_F24");
   if (storage != ctx->storage) ctx->hist_version = -1;
   ctx->storage = storage;
   return BORN_OK;
@@ -1599,6 +1639,8 @@ int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_pe
     }
     return fail(ctx, BORN_ERR_ARG, "no such launch shape");
   }
+  if (one_cell(kernel) && cells_per_lane != 1)
+    return fail(ctx, BORN_ERR_ARG, "24-bit storage kernels take one cell per lane");
   for (int sh = 0; sh < born::kNumShapes; sh++) {
     const born::TileShape &ts = born::kShapes[sh];
     if (ts.ry == rows_per_thread && ts.wy == waves_per_block && ts.vx == cells_per_lane &&
@@ -1648,12 +1690,29 @@ struct born_batch {
   int max_nrec = 0;
   float *d_rings = nullptr;
   int *d_pts = nullptr, *d_slot = nullptr, *d_rmap = nullptr, *d_rnext = nullptr;
-  born::Tune tune[4];
-  float tune_ms[4];
+  // per storage form (F32, F16, F24): forward (scatter 0, 1), background
+  // recompute (scatter 0, 1), adjoint state
+  born::Tune tune[18];
+  float tune_ms[18];
+  size_t f_cap = 0;    // floats per field buffer (24-bit levels: 6 bytes per cell)
+  long long max_shot_cells = 0;
+  std::vector<float> vmin2;  // per shot: min of V over the stencil interior
+  float bmin = 1.f;          // smallest sponge factor (the sponge of the batch)
+  size_t acc_cap = 0, img_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int g_jt = -1, g_nt = -1, g_scatter = -1, g_storage = -1, g_shape = -1, g_zc = -1;
   long long g_tab = -1;  // ctx->tab_version the graph was captured with
+
+  // exact adjoint: accumulator and image (cells floats), background history
+  // and checkpoints (bytes), per-level scales [L][nshots] and checkpointed
+  // s rings [S - 1][nshots][8] of scaled storage
+  float *d_acc = nullptr, *d_img = nullptr;
+  float *d_seg = nullptr, *d_ckpt = nullptr;
+  size_t seg_bytes = 0, ckpt_bytes = 0;
+  float *d_hscale = nullptr, *d_ckring = nullptr;
+  size_t hscale_cap = 0, ckring_cap = 0;
+  int last_segments = 0, last_recomputed = 0;
 };
 
 namespace {
@@ -1678,6 +1737,12 @@ void batch_free(born_batch *b) {
   (void)dev_alloc(ctx, &b->d_rmap, 0);
   (void)dev_alloc(ctx, &b->d_rnext, 0);
   (void)dev_alloc(ctx, &b->d_desc, 0);
+  (void)dev_alloc(ctx, &b->d_acc, 0);
+  (void)dev_alloc(ctx, &b->d_img, 0);
+  (void)dev_alloc(ctx, &b->d_seg, 0);
+  (void)dev_alloc(ctx, &b->d_ckpt, 0);
+  (void)dev_alloc(ctx, &b->d_hscale, 0);
+  (void)dev_alloc(ctx, &b->d_ckring, 0);
 }
 
 template <class T>
@@ -1699,6 +1764,9 @@ int batch_build(born_batch *b, int nshots, const born_shot *shots) {
   std::vector<float> bound(ctx->nbound);
   for (int i = 0; i < ctx->nbound; i++)
     bound[i] = expf(-ctx->bcB * ((float)ctx->nbound - i));
+  b->bmin = 1.f;
+  for (float v : bound) b->bmin = std::min(b->bmin, v);
+  b->vmin2.clear();
   long long cells = 0, tr = 0;
   for (int s = 0; s < nshots; s++) {
     const born_shot &sh = shots[s];
@@ -1721,7 +1789,10 @@ int batch_build(born_batch *b, int nshots, const born_shot *shots) {
     d.nz = sh.nz;
     d.npts = sh.npts;
     d.cell0 = cells;
-    cells += n + (n & 1);  // keep every shot's first cell at an even offset
+    // every shot's first cell at a multiple of 64: whole blocks of 24-bit
+    // levels, and an even offset for the two-cell-per-lane tiles
+    cells += (n + 63) / 64 * 64;
+    b->max_shot_cells = std::max(b->max_shot_cells, n);
     d.sp0 = (int)sponge.size();
     for (int ax = 0; ax < 3; ax++) {
       const int na = ax == 0 ? sh.nx : ax == 1 ? sh.ny : sh.nz;
@@ -1752,9 +1823,18 @@ int batch_build(born_batch *b, int nshots, const born_shot *shots) {
     d.tr0 = tr;
     tr += (long long)nrec * b->ntb;
     d.ring0 = s * born::kRingWords;
-    float vmax = 0.f;
-    for (long long i = 0; i < n; i++) vmax = std::max(vmax, fabsf(sh.vel[i]));
+    float vmax = 0.f, vmin = INFINITY;
+    for (long long i = 0; i < n; i++) {
+      vmax = std::max(vmax, fabsf(sh.vel[i]));
+      const int x = (int)(i % sh.nx), y = (int)((i / sh.nx) % sh.ny);
+      const int z = (int)(i / ((long long)sh.nx * sh.ny));
+      if (x >= 4 && x < sh.nx - 4 && y >= 4 && y < sh.ny - 4 && z >= 4 && z < sh.nz - 4)
+        vmin = std::min(vmin, sh.vel[i]);
+    }
     d.gain = 2.f + vmax * b->tapsum;
+    d.vmax = vmax;
+    d.per_cell = rt.per_cell;
+    b->vmin2.push_back(vmin);
   }
   b->cells = cells;
   b->tr_total = tr;
@@ -1762,6 +1842,7 @@ int batch_build(born_batch *b, int nshots, const born_shot *shots) {
   CHK(set_device(ctx));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &b->d_f[i], (size_t)cells));
+  b->f_cap = (size_t)cells;
   CHK(dev_alloc(ctx, &b->d_vel, (size_t)cells));
   CHK(dev_alloc(ctx, &b->d_model, (size_t)cells));
   for (int s = 0; s < nshots; s++) {
@@ -1780,7 +1861,7 @@ int batch_build(born_batch *b, int nshots, const born_shot *shots) {
   CHK(dev_alloc(ctx, &b->d_tr, (size_t)std::max(tr, 1ll)));
   CHK(dev_alloc(ctx, &b->d_tmp, (size_t)b->max_nrec * b->ntb));
   CHK(dev_alloc(ctx, &b->d_rings, (size_t)nshots * born::kRingWords));
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < 18; k++) {
     b->tune[k] = {3, 64};
     b->tune_ms[k] = -1.f;
   }
@@ -1788,7 +1869,7 @@ int batch_build(born_batch *b, int nshots, const born_shot *shots) {
 }
 
 // the step launch of the batch at time step it
-born::BatchFwdStep batch_step(born_batch *b, int it, int jt, int scatter, bool half) {
+born::BatchFwdStep batch_step(born_batch *b, int it, int jt, int scatter, int storage) {
   born_ctx *ctx = b->ctx;
   born::BatchFwdStep st{};
   born::FwdStep &a = st.a;
@@ -1812,7 +1893,7 @@ born::BatchFwdStep batch_step(born_batch *b, int it, int jt, int scatter, bool h
   a.id = id;
   a.extract = id + 7 < b->ntb;
   a.scatter = scatter;
-  a.h.on = half;
+  a.h.on = storage;
   a.h.sl = slots(it);
   a.h.tapsum = b->tapsum;
   st.sponge = b->d_sponge;
@@ -1871,25 +1952,41 @@ int born_batch_forward(born_batch *b, int jt, int nt, int scatter) {
   if (nt < 0 || (scatter != 0 && scatter != 1))
     return fail(ctx, BORN_ERR_ARG, "bad forward arguments");
   CHK(set_device(ctx));
-  const bool half = ctx->storage == BORN_STORAGE_F16;
-  const int kind = (half ? 2 : 0) + scatter;
+  const int storage = ctx->storage;
+  const bool half = storage != BORN_STORAGE_F32;  // scaled 16-bit levels
+  const bool split = storage == BORN_STORAGE_F24;
+  const int kind = 6 * storage + scatter;
+  if (split && (unsigned long long)b->max_shot_cells * 6 >= (1ull << 32))
+    return fail(ctx, BORN_ERR_UNSUPPORTED,
+                "24-bit storage needs fewer than 715,827,882 cells per shot");
+  // 24-bit levels of two fields need 6 bytes per cell
+  const size_t fneed = split ? ((size_t)b->cells * 3 + 1) / 2 : (size_t)b->cells;
+  if (b->f_cap < fneed) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);  // the graph holds the old buffers
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->f_cap = 0;
+    for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &b->d_f[i], fneed));
+    b->f_cap = fneed;
+  }
   const born::Taps taps = taps_of(b->coeffs);
   CHK(ensure(ctx, &b->d_amp, &b->amp_cap,
              (size_t)std::max(1, b->npts_total) * std::max(nt, 1)));
   {  // tune on the batch arrays (cleared below), no traces written
     HIPCHK(born::launch_batch_ring_init(b->d_rings, b->nshots, ctx->stream));
-    born::BatchFwdStep t = batch_step(b, 0, jt, scatter, half);
+    born::BatchFwdStep t = batch_step(b, 0, jt, scatter, storage);
     t.a.extract = 0;
-    CHK(autotune_into(ctx, &b->tune[kind], &b->tune_ms[kind], b->even_nx, 0,
+    CHK(autotune_into(ctx, &b->tune[kind], &b->tune_ms[kind], b->even_nx && !split, 0,
                       [&](const born::Tune &tu) {
                         return born::launch_batch_fwd_step(taps, tu, t, ctx->stream);
                       }));
   }
   born::Tune tu = b->tune[kind];
   if (ctx->zchunk > 0) tu.zchunk = ctx->zchunk;
-  if (born::kShapes[tu.shape].vx == 2 && !b->even_nx) tu.shape = 3;
+  if (born::kShapes[tu.shape].vx == 2 && (!b->even_nx || split)) tu.shape = 3;
   // initial state: fields and traces zero, amplitudes of every shot, scales
-  for (int i = 0; i < 4; i++) CHK(zero(ctx, b->d_f[i], b->cells));
+  for (int i = 0; i < 4; i++) CHK(zero(ctx, b->d_f[i], (long long)b->f_cap));
   CHK(zero(ctx, b->d_tr, b->tr_total));
   for (int s = 0; s < b->nshots; s++) {
     const born::ShotDesc &d = b->desc[s];
@@ -1919,7 +2016,7 @@ int born_batch_forward(born_batch *b, int jt, int nt, int scatter) {
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     hipError_t le = hipSuccess;
     for (int it = 0; it < nt && le == hipSuccess; it++)
-      le = born::launch_batch_fwd_step(taps, tu, batch_step(b, it, jt, scatter, half),
+      le = born::launch_batch_fwd_step(taps, tu, batch_step(b, it, jt, scatter, storage),
                                        ctx->stream);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
@@ -1956,10 +2053,313 @@ int born_batch_traces(born_batch *b, int shot, float *rec) {
   return BORN_OK;
 }
 
+int born_batch_set_traces(born_batch *b, int shot, const float *rec) {
+  if (!b || !rec) return BORN_ERR_ARG;
+  born_ctx *ctx = b->ctx;
+  if (shot < 0 || shot >= b->nshots) return fail(ctx, BORN_ERR_ARG, "no such shot");
+  CHK(set_device(ctx));
+  const born::ShotDesc &d = b->desc[shot];
+  HIPCHK(hipMemcpyAsync(b->d_tmp, rec, (size_t)d.nrec * b->ntb * sizeof(float),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(born::launch_transpose(b->d_tr + d.tr0, b->d_tmp, d.nrec, b->ntb, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BORN_OK;
+}
+
+int born_batch_adjoint_exact(born_batch *b, int jt, int nt, int scatter) {
+  if (!b) return BORN_ERR_ARG;
+  born_ctx *ctx = b->ctx;
+  if (!ctx->d_tab_s || !ctx->d_tab_d)
+    return fail(ctx, BORN_ERR_STATE, "sinc tables have not been transferred");
+  CHK(check_jt(ctx, jt));
+  if (nt < 0 || (scatter != 0 && scatter != 1))
+    return fail(ctx, BORN_ERR_ARG, "bad adjoint arguments");
+  for (int s = 0; s < b->nshots; s++)
+    if (!(b->vmin2[s] * b->bmin > 0.f))
+      return fail(ctx, BORN_ERR_UNSUPPORTED,
+                  "shot " + std::to_string(s) +
+                      ": the exact adjoint needs the velocity and the sponge factors > 0 "
+                      "in the interior");
+  CHK(set_device(ctx));
+  const int storage = ctx->storage;
+  const bool half = storage != BORN_STORAGE_F32;  // scaled 16-bit levels
+  const bool split = storage == BORN_STORAGE_F24;
+  if (split && (unsigned long long)b->max_shot_cells * 6 >= (1ull << 32))
+    return fail(ctx, BORN_ERR_UNSUPPORTED,
+                "24-bit storage needs fewer than 715,827,882 cells per shot");
+  const size_t esz = elem(ctx);
+  const long long cells = b->cells;
+  const size_t fl = ((size_t)cells * esz + 3) / 4;  // floats of one single-field level
+  CHK(ensure(ctx, &b->d_acc, &b->acc_cap, (size_t)cells));
+  CHK(ensure(ctx, &b->d_img, &b->img_cap, (size_t)cells));
+  CHK(zero(ctx, b->d_img, cells));
+  if (nt == 0) return BORN_OK;
+  CHK(ensure(ctx, &b->d_amp, &b->amp_cap,
+             (size_t)std::max(1, b->npts_total) * std::max(nt, 1)));
+  const born::Taps taps = taps_of(b->coeffs);
+  const int skind = 6 * storage + 2 + scatter, lkind = 6 * storage + 4;
+  auto common = [&](born::BatchCommon &c, int step) {
+    c.npts_total = b->npts_total;
+    c.sponge = b->d_sponge;
+    c.rings = b->d_rings;
+    c.desc = b->d_desc;
+    c.host_desc = b->desc.data();
+    c.nshots = b->nshots;
+    c.step = step;
+  };
+  auto hstate = [&](int k) {
+    born::Half h;
+    h.on = storage;
+    h.sl = slots(k);
+    h.tapsum = b->tapsum;
+    h.gain = 0.f;  // per shot (ShotDesc::gain)
+    return h;
+  };
+  auto tune_of = [&](int kind) {
+    born::Tune t = b->tune[kind];
+    if (ctx->zchunk > 0) t.zchunk = ctx->zchunk;
+    if (born::kShapes[t.shape].vx == 2 && (!b->even_nx || split)) t.shape = 3;
+    return t;
+  };
+  float *s_old = b->d_f[2], *s_cur = b->d_f[3];
+  float *u_old = b->d_f[0], *u_cur = b->d_f[1];
+  {  // tune both kernels on the batch fields and scratch scales (rings reset below)
+    HIPCHK(born::launch_batch_ring_init(b->d_rings, b->nshots, ctx->stream));
+    HIPCHK(born::launch_batch_ring_reset(b->d_rings, b->nshots, born::BR_U, 0, ctx->stream));
+    CHK(ensure(ctx, &b->d_hscale, &b->hscale_cap, (size_t)b->nshots));
+    HIPCHK(born::launch_fill(b->d_hscale, b->nshots, 1.f, ctx->stream));
+    born::BatchSrecStep t{};
+    common(t, 0);
+    t.a.s1 = s_cur; t.a.s0 = s_old; t.a.vel = b->d_vel; t.a.sig = b->d_acc;
+    t.a.src = {b->d_pts, 0}; t.a.amp = b->d_amp; t.a.scatter = scatter;
+    t.a.h = hstate(0); t.a.hscale = b->d_hscale;
+    CHK(autotune_into(ctx, &b->tune[skind], &b->tune_ms[skind], b->even_nx && !split, 0,
+                      [&](const born::Tune &tu) {
+                        return born::launch_batch_srec_step(taps, tu, t, ctx->stream);
+                      }));
+    born::BatchLamStep l{};
+    common(l, 0);
+    l.a.u1 = u_cur; l.a.u0 = u_old; l.a.vel = b->d_vel; l.a.sig = s_old; l.a.acc = b->d_acc;
+    l.a.rec.plane_slot = b->d_slot; l.a.rec.rmap = b->d_rmap; l.a.rec.rnext = b->d_rnext;
+    l.a.rec.ntb = b->ntb; l.a.tr = b->d_tr; l.a.tab = ctx->d_tab_d; l.a.inject = 0;
+    l.a.h = hstate(0); l.a.inj = ctx->tabd_abs; l.a.hscale = b->d_hscale;
+    CHK(autotune_into(ctx, &b->tune[lkind], &b->tune_ms[lkind], b->even_nx && !split, 0,
+                      [&](const born::Tune &tu) {
+                        return born::launch_batch_lam_step(taps, tu, l, ctx->stream);
+                      }));
+  }
+  const born::Tune stu = tune_of(skind), ltu = tune_of(lkind);
+  // checkpoint plan: the fewest segments S with (L + 2 (S - 1)) levels in the
+  // budget, L = ceil(nt / S) (every shot shares the plan)
+  const size_t F = (size_t)cells * esz;
+  size_t avail = ctx->budget;
+  if (!avail) {
+    const size_t fr = free_bytes(ctx) + b->seg_bytes + b->ckpt_bytes;
+    const size_t reserve = (size_t)2 << 30;
+    avail = fr > reserve ? fr - reserve : 0;
+  }
+  int S = 0;
+  for (int sg = 1; sg <= nt; sg++) {
+    const size_t l = (size_t)((nt + sg - 1) / sg);
+    if ((l + 2 * (size_t)(sg - 1)) * F <= avail) {
+      S = sg;
+      break;
+    }
+    if (l <= 2) break;
+  }
+  if (S == 0) return fail(ctx, BORN_ERR_NOMEM, "no checkpoint plan fits the HBM budget");
+  const int L = (nt + S - 1) / S;
+  S = (nt + L - 1) / L;
+  if (b->seg_bytes < (size_t)L * F) {
+    b->seg_bytes = 0;
+    CHK(dev_alloc(ctx, &b->d_seg, ((size_t)L * F + 3) / 4));
+    b->seg_bytes = (size_t)L * F;
+  }
+  if (b->ckpt_bytes < 2 * (size_t)(S - 1) * F) {
+    b->ckpt_bytes = 0;
+    CHK(dev_alloc(ctx, &b->d_ckpt, (2 * (size_t)(S - 1) * F + 3) / 4));
+    b->ckpt_bytes = 2 * (size_t)(S - 1) * F;
+  }
+  if (half) {
+    CHK(ensure(ctx, &b->d_hscale, &b->hscale_cap, (size_t)L * b->nshots));
+    CHK(ensure(ctx, &b->d_ckring, &b->ckring_cap, 8 * (size_t)b->nshots * std::max(S - 1, 1)));
+  }
+  // amplitudes of every shot, as the forward (src_amp per shot)
+  for (int s = 0; s < b->nshots; s++) {
+    const born::ShotDesc &d = b->desc[s];
+    if (d.npts)
+      HIPCHK(born::launch_src_amp(b->d_amp + d.pts0, nt, d.npts, b->d_src + b->src0[s], b->ntb,
+                                  ctx->d_tab_s, ctx->nsinc_s, jt, ctx->jts, 0, 1 / (float)jt,
+                                  ctx->stream, b->npts_total));
+  }
+  auto lev = [&](float *base, long long k) { return nth(base, k, cells, esz); };
+  // the 8 s-ring words of every shot to / from checkpoint c
+  auto ring_copy = [&](long long c, bool save) -> int {
+    float *ck = b->d_ckring + 8 * (size_t)b->nshots * c;
+    float *rg = b->d_rings + born::BR_S;
+    HIPCHK(hipMemcpy2DAsync(save ? ck : rg, (save ? 8 : born::kRingWords) * sizeof(float),
+                            save ? rg : ck, (save ? born::kRingWords : 8) * sizeof(float),
+                            8 * sizeof(float), b->nshots, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    return BORN_OK;
+  };
+  const bool s_in_seg = scatter == 0;
+  born::BatchSrecStep sr{};
+  common(sr, 0);
+  sr.a.vel = b->d_vel;
+  sr.a.src = {b->d_pts, 0};
+  sr.a.amp = b->d_amp;
+  sr.a.scatter = scatter;
+  auto srec = [&](int k, float *seg_level, float *sig_level, float *s1, float *s0,
+                  float *s_new) -> int {
+    sr.step = k;
+    sr.a.s1 = s1;
+    sr.a.s0 = s0;
+    sr.a.s_new = s_new;
+    sr.a.sig = sig_level;
+    sr.a.hscale = half && seg_level ? b->d_hscale + (size_t)b->nshots * (k % L) : nullptr;
+    sr.a.h = hstate(k);
+    HIPCHK(born::launch_batch_srec_step(taps, stu, sr, ctx->stream));
+    return BORN_OK;
+  };
+  // pass 1: background sweep; checkpoints at segment starts (with the s
+  // rings), the last segment's scatter operands into the segment history
+  CHK(zero(ctx, s_old, (long long)fl));
+  CHK(zero(ctx, s_cur, (long long)fl));
+  if (half) HIPCHK(born::launch_batch_ring_reset(b->d_rings, b->nshots, born::BR_S, 0, ctx->stream));
+  const int klast = (S - 1) * L;
+  for (int k = 0; k < nt; k++) {
+    if (k > 0 && k % L == 0 && k / L <= S - 1) {
+      const long long c = k / L - 1;
+      HIPCHK(hipMemcpyAsync(lev(b->d_ckpt, 2 * c), s_cur, F, hipMemcpyDeviceToDevice, ctx->stream));
+      HIPCHK(hipMemcpyAsync(lev(b->d_ckpt, 2 * c + 1), s_old, F, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+      if (half) CHK(ring_copy(c, true));
+    }
+    float *sig = k >= klast ? lev(b->d_seg, k - klast) : nullptr;
+    if (s_in_seg && k >= klast) {
+      // levels klast and klast-1 stay in s_cur and s_old; level j > klast is
+      // in segment level j-1-klast
+      float *s1 = k > klast ? lev(b->d_seg, k - 1 - klast) : s_cur;
+      float *s0 = k > klast + 1 ? lev(b->d_seg, k - 2 - klast) : (k > klast ? s_cur : s_old);
+      CHK(srec(k, sig, nullptr, s1, s0, sig));
+    } else {
+      CHK(srec(k, sig, sig, s_cur, s_old, nullptr));
+      std::swap(s_old, s_cur);
+    }
+  }
+  // pass 2: adjoint state, segment by segment in reverse
+  CHK(zero(ctx, u_old, (long long)fl));
+  CHK(zero(ctx, u_cur, (long long)fl));
+  CHK(zero(ctx, b->d_acc, cells));
+  if (half) {
+    HIPCHK(born::launch_batch_ring_reset(b->d_rings, b->nshots, born::BR_U, 0, ctx->stream));
+    HIPCHK(born::launch_batch_ring_reset(b->d_rings, b->nshots, born::BR_MAXTR, 1, ctx->stream));
+    for (int s = 0; s < b->nshots; s++) {
+      const born::ShotDesc &d = b->desc[s];
+      HIPCHK(born::launch_absmax(b->d_tr + d.tr0, (long long)d.nrec * b->ntb,
+                                 (unsigned *)(b->d_rings + d.ring0 + born::BR_MAXTR),
+                                 ctx->stream));
+    }
+  }
+  born::BatchLamStep lm{};
+  common(lm, 0);
+  lm.a.vel = b->d_vel;
+  lm.a.acc = b->d_acc;
+  lm.a.rec.plane_slot = b->d_slot;
+  lm.a.rec.rmap = b->d_rmap;
+  lm.a.rec.rnext = b->d_rnext;
+  lm.a.rec.ntb = b->ntb;
+  lm.a.tr = b->d_tr;
+  lm.a.inj = ctx->tabd_abs;
+  int step = 0, recomputed = 0;
+  for (int sg = S - 1; sg >= 0; sg--) {
+    const int k0 = sg * L, k1 = std::min(nt, k0 + L);
+    if (sg < S - 1) {
+      if (sg == 0) {
+        CHK(zero(ctx, s_old, (long long)fl));
+        CHK(zero(ctx, s_cur, (long long)fl));
+        if (half)
+          HIPCHK(born::launch_batch_ring_reset(b->d_rings, b->nshots, born::BR_S, 0, ctx->stream));
+      } else {
+        const long long c = sg - 1;
+        HIPCHK(hipMemcpyAsync(s_cur, lev(b->d_ckpt, 2 * c), F, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        HIPCHK(hipMemcpyAsync(s_old, lev(b->d_ckpt, 2 * c + 1), F, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        if (half) CHK(ring_copy(c, false));
+      }
+      for (int k = k0; k < k1; k++) {
+        float *sig = lev(b->d_seg, k - k0);
+        if (s_in_seg) {
+          float *s1 = k > k0 ? lev(b->d_seg, k - 1 - k0) : s_cur;
+          float *s0 = k > k0 + 1 ? lev(b->d_seg, k - 2 - k0) : (k > k0 ? s_cur : s_old);
+          CHK(srec(k, sig, nullptr, s1, s0, sig));
+        } else {
+          CHK(srec(k, sig, sig, s_cur, s_old, nullptr));
+          std::swap(s_old, s_cur);
+        }
+        recomputed++;
+      }
+    }
+    for (int k = k1 - 1; k >= k0; k--) {
+      const int id = k / jt, ii = k - id * jt;
+      lm.step = k;
+      lm.a.u1 = u_cur;
+      lm.a.u0 = u_old;
+      lm.a.sig = lev(b->d_seg, k - k0);
+      lm.a.hscale = half ? b->d_hscale + (size_t)b->nshots * (k % L) : nullptr;
+      lm.a.tab = ctx->d_tab_d + (long long)ii * 8;
+      lm.a.id = id;
+      lm.a.inject = id + 7 < b->ntb;
+      lm.a.h = hstate(step++);
+      HIPCHK(born::launch_batch_lam_step(taps, ltu, lm, ctx->stream));
+      std::swap(u_old, u_cur);
+    }
+  }
+  // image of every shot: acc / (V B), times 1e-6 for the reference's scatter
+  for (int s = 0; s < b->nshots; s++) {
+    const born::ShotDesc &d = b->desc[s];
+    Geometry g{};
+    g.nx = d.nx;
+    g.ny = d.ny;
+    g.nz = d.nz;
+    g.sponge.bx = b->d_sponge + d.sp0;
+    g.sponge.by = b->d_sponge + d.sp0 + d.nx;
+    g.sponge.bz = b->d_sponge + d.sp0 + d.nx + d.ny;
+    HIPCHK(born::launch_adj_finish(g, b->d_img + d.cell0, b->d_acc + d.cell0,
+                                   b->d_vel + d.cell0, scatter == 0 ? 1e-6f : 1.f,
+                                   ctx->stream));
+  }
+  b->last_segments = S;
+  b->last_recomputed = recomputed;
+  return BORN_OK;
+}
+
+int born_batch_image(born_batch *b, int shot, float *img) {
+  if (!b || !img) return BORN_ERR_ARG;
+  born_ctx *ctx = b->ctx;
+  if (shot < 0 || shot >= b->nshots) return fail(ctx, BORN_ERR_ARG, "no such shot");
+  if (!b->d_img) return fail(ctx, BORN_ERR_STATE, "no batch adjoint has run");
+  CHK(set_device(ctx));
+  const born::ShotDesc &d = b->desc[shot];
+  HIPCHK(hipMemcpyAsync(img, b->d_img + d.cell0, (size_t)d.nx * d.ny * d.nz * sizeof(float),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BORN_OK;
+}
+
+int born_batch_last_checkpoint_plan(const born_batch *b, int *segments, int *recomputed_steps) {
+  if (!b) return BORN_ERR_ARG;
+  if (segments) *segments = b->last_segments;
+  if (recomputed_steps) *recomputed_steps = b->last_recomputed;
+  return BORN_OK;
+}
+
 int born_batch_get_tile(const born_batch *b, int scatter, int *tile_x, int *tile_y,
                         int *zchunk, float *ms) {
   if (!b || (scatter != 0 && scatter != 1)) return BORN_ERR_ARG;
-  const int kind = (b->ctx->storage == BORN_STORAGE_F16 ? 2 : 0) + scatter;
+  const int kind = 6 * b->ctx->storage + scatter;
   const born::Tune t = b->tune[kind];
   const born::TileShape &ts = born::kShapes[t.shape];
   if (tile_x) *tile_x = ts.lx * ts.vx;

@@ -90,6 +90,15 @@ __device__ __forceinline__ const T &at(const T *p, uint32_t o) {
 // step reads a new level back before the next launch, and the hint took 6 %
 // off the FP32 forward step at 304^3.
 template <bool NT>
+__device__ __forceinline__ void st_b8(void *p, uint32_t off, unsigned char v) {
+  if constexpr (NT)
+    asm volatile("s_nop 4\n\tglobal_store_byte %0, %1, %2 nt"
+                 : : "v"(off), "v"((uint32_t)v), "s"(p) : "memory");
+  else
+    asm volatile("s_nop 4\n\tglobal_store_byte %0, %1, %2"
+                 : : "v"(off), "v"((uint32_t)v), "s"(p) : "memory");
+}
+template <bool NT>
 __device__ __forceinline__ void st_b16(void *p, uint32_t off, unsigned short v) {
   if constexpr (NT)
     asm volatile("s_nop 4\n\tglobal_store_short %0, %1, %2 nt"
@@ -309,7 +318,7 @@ __device__ __forceinline__ float sat_half(float v) { return fminf(fmaxf(v, -6550
 // separate FP16 arrays.  inv = 1/scale of each field.
 template <int VX>
 __device__ __forceinline__ void ld_pair(const __half2 *p, uint32_t o, const float (&inv)[2],
-                                        float (&v)[2][VX]) {
+                                        float (&v)[2][VX], uint32_t = 0) {
   if constexpr (VX == 1) {
     const __half2 w = at(p, o);
     v[0][0] = __low2float(w) * inv[0];
@@ -327,7 +336,8 @@ __device__ __forceinline__ void ld_pair(const __half2 *p, uint32_t o, const floa
 }
 template <int VX, bool NT = true>
 __device__ __forceinline__ void st_pair(__half2 *p, uint32_t o, const float (&a)[VX],
-                                        const float (&b)[VX], const float (&scale)[2]) {
+                                        const float (&b)[VX], const float (&scale)[2],
+                                        uint32_t = 0) {
   const __half2 c0 = __floats2half2_rn(sat_half(a[0] * scale[0]), sat_half(b[0] * scale[1]));
   uint32_t u0;
   __builtin_memcpy(&u0, &c0, 4);
@@ -341,11 +351,106 @@ __device__ __forceinline__ void st_pair(__half2 *p, uint32_t o, const float (&a)
   }
 }
 
+// ---- 24-bit storage: an FP16 value plus a refinement byte per cell.  The
+// level is scaled as in FP16 storage (|value * scale| < 16384 by the level's
+// bound).  w = value * scale is rounded to 19 significant bits (half-way
+// cases away from zero); hi holds its top 11 bits as an FP16 value and lo the
+// next 8 bits of the FP32 significand, so hi's FP32 bits with lo shifted in
+// below them are the rounded w.  The relative precision is 2^-19 at every
+// magnitude (FP16 alone: 2^-11), so the rounding noise injected at every step
+// no longer dominates the error of thousands of steps.  (Below FP16's normal
+// range, 2^-27 of the bound, hi is rounded toward zero and lo perturbs it by
+// less than 2^-10 of itself.)
+//
+// Layout of a level of n cells: the n hi values, then the n lo bytes; for two
+// fields stored together (s, r), n __half2 then n byte pairs.  So a cell's
+// hi and lo addresses are a shift of its offset, lo's plus the uniform
+// 2n (4n) bytes.  A pointer to Split (SplitPair) advances by 3 (6) bytes per
+// cell: the levels of a history and the shots of a batch are whole regions
+// of 3n (6n) bytes.  One cell per lane only (VX = 1).
+struct Split {
+  unsigned char b[3];
+};
+struct SplitPair {
+  unsigned char b[6];
+};
+static_assert(sizeof(Split) == 3 && sizeof(SplitPair) == 6, "packed split types");
+
+// value of one stored cell: (hi with lo's 8 bits below it) / scale
+__device__ __forceinline__ float split_val(uint32_t hi_f32_bits, uint32_t lo, float inv) {
+  return mul_r(__uint_as_float(hi_f32_bits | (lo & 0xffu) << 5), inv);
+}
+// w = value * scale rounded to 19 bits: the 11-bit top part (FP16-exact)
+// and the byte below it
+__device__ __forceinline__ float split_top(float v, float scale, uint32_t &lo) {
+  const float w = fminf(fmaxf(mul_r(v, scale), -65504.f), 65504.f);
+  const uint32_t q = __float_as_uint(w) + 0x10u;  // round at the 19th bit
+  lo = (q >> 5) & 0xffu;
+  return __uint_as_float(q & 0xffffe000u);
+}
+__device__ __forceinline__ uint32_t f16_f32_bits(unsigned short h) {
+  return __float_as_uint(__half2float(__ushort_as_half(h)));
+}
+
+template <int VX>
+__device__ __forceinline__ void ld_vec(const Split *p, uint32_t o, float inv, float (&v)[VX],
+                                       uint32_t n) {
+  static_assert(VX == 1, "24-bit storage: one cell per lane");
+  const char *b = (const char *)p;
+  const unsigned short hi = *(const unsigned short *)(b + o * 2u);
+  const uint32_t lo = *(const unsigned char *)(b + (2u * n + o));
+  v[0] = split_val(f16_f32_bits(hi), lo, inv);
+}
+template <int VX>
+__device__ __forceinline__ void ld_stream(const Split *p, uint32_t o, float inv,
+                                          float (&v)[VX], uint32_t n) {
+  ld_vec<VX>(p, o, inv, v, n);
+}
+template <int VX, bool NT = true>
+__device__ __forceinline__ void st_vec(Split *p, uint32_t o, const float (&v)[VX], float scale,
+                                       uint32_t n) {
+  static_assert(VX == 1, "24-bit storage: one cell per lane");
+  uint32_t lo;
+  const float top = split_top(v[0], scale, lo);
+  // top is FP16-exact where it matters; round toward zero below FP16's range
+  const auto h = __builtin_amdgcn_cvt_pkrtz(top, 0.f);
+  uint32_t hb;
+  __builtin_memcpy(&hb, &h, 4);
+  st_b16<NT>(p, o * 2u, (unsigned short)(hb & 0xffffu));
+  st_b8<NT>(p, 2u * n + o, (unsigned char)lo);
+}
+// two fields stored together: first field in the low half of each pair
+template <int VX>
+__device__ __forceinline__ void ld_pair(const SplitPair *p, uint32_t o, const float (&inv)[2],
+                                        float (&v)[2][VX], uint32_t n) {
+  static_assert(VX == 1, "24-bit storage: one cell per lane");
+  const char *b = (const char *)p;
+  const uint32_t hi = *(const uint32_t *)(b + o * 4u);
+  const uint32_t lo = *(const unsigned short *)(b + (4u * n + o * 2u));
+  v[0][0] = split_val(f16_f32_bits((unsigned short)(hi & 0xffffu)), lo, inv[0]);
+  v[1][0] = split_val(f16_f32_bits((unsigned short)(hi >> 16)), lo >> 8, inv[1]);
+}
+template <int VX, bool NT = true>
+__device__ __forceinline__ void st_pair(SplitPair *p, uint32_t o, const float (&a)[VX],
+                                        const float (&b)[VX], const float (&scale)[2],
+                                        uint32_t n) {
+  static_assert(VX == 1, "24-bit storage: one cell per lane");
+  uint32_t l0, l1;
+  const float t0 = split_top(a[0], scale[0], l0);
+  const float t1 = split_top(b[0], scale[1], l1);
+  const auto h = __builtin_amdgcn_cvt_pkrtz(t0, t1);
+  uint32_t hb;
+  __builtin_memcpy(&hb, &h, 4);
+  st_b32<NT>(p, o * 4u, hb);
+  st_b16<NT>(p, 4u * n + o * 2u, (unsigned short)(l0 | l1 << 8));
+}
+
 // ---- VX consecutive cells of a row (VX = 1 or 2).  A pair starts at an
 // even element offset (x tiles start at even x and nx is even when VX = 2),
 // so float2 / __half2 accesses are naturally aligned.
 template <int VX>
-__device__ __forceinline__ void ld_vec(const float *p, uint32_t o, float, float (&v)[VX]) {
+__device__ __forceinline__ void ld_vec(const float *p, uint32_t o, float, float (&v)[VX],
+                                       uint32_t = 0) {
   if constexpr (VX == 1) {
     v[0] = at(p, o);
   } else {
@@ -356,7 +461,7 @@ __device__ __forceinline__ void ld_vec(const float *p, uint32_t o, float, float 
 }
 template <int VX>
 __device__ __forceinline__ void ld_vec(const __half *p, uint32_t o, float inv,
-                                       float (&v)[VX]) {
+                                       float (&v)[VX], uint32_t = 0) {
   if constexpr (VX == 1) {
     v[0] = __half2float(at(p, o)) * inv;
   } else {
@@ -368,7 +473,8 @@ __device__ __forceinline__ void ld_vec(const __half *p, uint32_t o, float inv,
 // The same loads with the non-temporal hint, for inputs a launch reads once
 // (the pointwise operands of a cell).
 template <int VX>
-__device__ __forceinline__ void ld_stream(const float *p, uint32_t o, float, float (&v)[VX]) {
+__device__ __forceinline__ void ld_stream(const float *p, uint32_t o, float, float (&v)[VX],
+                                          uint32_t = 0) {
   const float *q = (const float *)((const char *)p + (uint64_t)(o * 4u));
   if constexpr (VX == 1) {
     v[0] = __builtin_nontemporal_load(q);
@@ -383,12 +489,13 @@ __device__ __forceinline__ void ld_stream(const float *p, uint32_t o, float, flo
 // and background steps ran 15-20 % slower (304^3, 2- and 4-byte lanes).
 template <int VX>
 __device__ __forceinline__ void ld_stream(const __half *p, uint32_t o, float inv,
-                                          float (&v)[VX]) {
+                                          float (&v)[VX], uint32_t = 0) {
   ld_vec<VX>(p, o, inv, v);
 }
 
 template <int VX, bool NT = true>
-__device__ __forceinline__ void st_vec(float *p, uint32_t o, const float (&v)[VX], float) {
+__device__ __forceinline__ void st_vec(float *p, uint32_t o, const float (&v)[VX], float,
+                                       uint32_t = 0) {
   if constexpr (VX == 1)
     st_b32<NT>(p, o * 4u, __float_as_uint(v[0]));
   else
@@ -396,7 +503,7 @@ __device__ __forceinline__ void st_vec(float *p, uint32_t o, const float (&v)[VX
 }
 template <int VX, bool NT = true>
 __device__ __forceinline__ void st_vec(__half *p, uint32_t o, const float (&v)[VX],
-                                       float scale) {
+                                       float scale, uint32_t = 0) {
   if constexpr (VX == 1) {
     st_b16<NT>(p, o * 2u, __half_as_ushort(__float2half_rn(sat_half(v[0] * scale))));
   } else {

@@ -14,6 +14,16 @@ __global__ void batch_ring_init_kernel(float *rings, int nshots) {
   rings[i] = (w < 4 || (w >= 8 && w < 12)) ? 1.f : 0.f;
 }
 
+// ring words [at, at + 4) = 1 and [at + 4, at + 8) = 0 of every shot; or,
+// with maxima_only, word at = 0
+__global__ void batch_ring_reset_kernel(float *rings, int nshots, int at, int maxima_only) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = maxima_only ? 1 : 8;
+  if (i >= nshots * per) return;
+  const int s = i / per, w = i % per;
+  rings[s * kRingWords + at + w] = (!maxima_only && w < 4) ? 1.f : 0.f;
+}
+
 // ring slots: scale 1, max 0
 __global__ void ring_init_kernel(float *scale, unsigned *maxb, int n) {
   const int i = threadIdx.x;
@@ -108,50 +118,71 @@ const TileShape kShapes[kNumShapes] = {BORN_TILES(1), BORN_TILES(2)};
 
 // the launchers of born_launch.h: dispatch on storage and scatter to the
 // instantiations of the born_k_*.hip units
-BORN_FWD_STEP(extern template, 0, float);
-BORN_FWD_STEP(extern template, 1, float);
-BORN_FWD_STEP(extern template, 0, __half);
-BORN_FWD_STEP(extern template, 1, __half);
-BORN_SREC_STEP(extern template, 0, float);
-BORN_SREC_STEP(extern template, 1, float);
-BORN_SREC_STEP(extern template, 0, __half);
-BORN_SREC_STEP(extern template, 1, __half);
-BORN_LAM_STEP(extern template, float);
-BORN_LAM_STEP(extern template, __half);
-BORN_BATCH_FWD_STEP(extern template, 0, float);
-BORN_BATCH_FWD_STEP(extern template, 1, float);
-BORN_BATCH_FWD_STEP(extern template, 0, __half);
-BORN_BATCH_FWD_STEP(extern template, 1, __half);
+#define BORN_ALL_TYPES(M, ...) M(__VA_ARGS__, float); M(__VA_ARGS__, __half); M(__VA_ARGS__, Split)
+BORN_ALL_TYPES(BORN_FWD_STEP, extern template, 0);
+BORN_ALL_TYPES(BORN_FWD_STEP, extern template, 1);
+BORN_ALL_TYPES(BORN_SREC_STEP, extern template, 0);
+BORN_ALL_TYPES(BORN_SREC_STEP, extern template, 1);
+BORN_ALL_TYPES(BORN_LAM_STEP, extern template);
+BORN_ALL_TYPES(BORN_BATCH_FWD_STEP, extern template, 0);
+BORN_ALL_TYPES(BORN_BATCH_FWD_STEP, extern template, 1);
+BORN_ALL_TYPES(BORN_BATCH_SREC_STEP, extern template, 0);
+BORN_ALL_TYPES(BORN_BATCH_SREC_STEP, extern template, 1);
+BORN_ALL_TYPES(BORN_BATCH_LAM_STEP, extern template);
+#undef BORN_ALL_TYPES
+
+// F<T>() for the storage of Half::on
+template <class F>
+static hipError_t by_storage(int on, F f) {
+  if (on == 2) return f(Split{});
+  if (on == 1) return f(__half{});
+  return f(0.f);
+}
 
 hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a,
                            hipStream_t st) {
-  if (a.h.on)
-    return a.scatter == 0 ? fwd_step<0, __half>(geo, tu, a, st) : fwd_step<1, __half>(geo, tu, a, st);
-  return a.scatter == 0 ? fwd_step<0, float>(geo, tu, a, st) : fwd_step<1, float>(geo, tu, a, st);
+  return by_storage(a.h.on, [&](auto t) {
+    using T = decltype(t);
+    return a.scatter == 0 ? fwd_step<0, T>(geo, tu, a, st) : fwd_step<1, T>(geo, tu, a, st);
+  });
 }
 
 hipError_t launch_srec_step(const Geometry &geo, const Tune &tu, const SrecStep &a,
                             hipStream_t st) {
-  if (a.h.on)
-    return a.scatter == 0 ? srec_step<0, __half>(geo, tu, a, st)
-                          : srec_step<1, __half>(geo, tu, a, st);
-  return a.scatter == 0 ? srec_step<0, float>(geo, tu, a, st) : srec_step<1, float>(geo, tu, a, st);
+  return by_storage(a.h.on, [&](auto t) {
+    using T = decltype(t);
+    return a.scatter == 0 ? srec_step<0, T>(geo, tu, a, st) : srec_step<1, T>(geo, tu, a, st);
+  });
 }
 
 hipError_t launch_lam_step(const Geometry &geo, const Tune &tu, const LamStep &a,
                            hipStream_t st) {
-  return a.h.on ? lam_step<__half>(geo, tu, a, st) : lam_step<float>(geo, tu, a, st);
+  return by_storage(a.h.on, [&](auto t) { return lam_step<decltype(t)>(geo, tu, a, st); });
 }
 
 hipError_t launch_batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
                                  hipStream_t st) {
-  if (b.a.h.on)
-    return b.a.scatter == 0 ? batch_fwd_step<0, __half>(taps, tu, b, st)
-                            : batch_fwd_step<1, __half>(taps, tu, b, st);
-  return b.a.scatter == 0 ? batch_fwd_step<0, float>(taps, tu, b, st)
-                          : batch_fwd_step<1, float>(taps, tu, b, st);
+  return by_storage(b.a.h.on, [&](auto t) {
+    using T = decltype(t);
+    return b.a.scatter == 0 ? batch_fwd_step<0, T>(taps, tu, b, st)
+                            : batch_fwd_step<1, T>(taps, tu, b, st);
+  });
 }
 
+hipError_t launch_batch_srec_step(const Taps &taps, const Tune &tu, const BatchSrecStep &b,
+                                  hipStream_t st) {
+  return by_storage(b.a.h.on, [&](auto t) {
+    using T = decltype(t);
+    return b.a.scatter == 0 ? batch_srec_step<0, T>(taps, tu, b, st)
+                            : batch_srec_step<1, T>(taps, tu, b, st);
+  });
+}
+
+hipError_t launch_batch_lam_step(const Taps &taps, const Tune &tu, const BatchLamStep &b,
+                                 hipStream_t st) {
+  return by_storage(b.a.h.on,
+                    [&](auto t) { return batch_lam_step<decltype(t)>(taps, tu, b, st); });
+}
 
 hipError_t launch_fill(float *p, long long n, float v, hipStream_t st) {
   if (n <= 0) return hipSuccess;
@@ -187,6 +218,15 @@ hipError_t launch_batch_ring_init(float *rings, int nshots, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(batch_ring_init_kernel, dim3((n + 255) / 256), dim3(256), 0, st, rings,
                      nshots);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_ring_reset(float *rings, int nshots, int at, int maxima_only,
+                                   hipStream_t st) {
+  const int n = nshots * (maxima_only ? 1 : 8);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_ring_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, st, rings,
+                     nshots, at, maxima_only);
   return hipGetLastError();
 }
 

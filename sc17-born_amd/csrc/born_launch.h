@@ -40,7 +40,7 @@ struct SrcPts {
 // FP16 storage of the step's wavefield levels (and of the history): the field
 // pointers then address __half arrays; see Ring in born_device.h.
 struct Half {
-  int on;
+  int on;  // 0: float levels; 1: FP16; 2: FP16 with a refinement byte (Split)
   Slots sl;
   float gain, tapsum;
 };
@@ -164,22 +164,42 @@ struct ShotDesc {
   int slot0;        // receiver plane slots (nz entries)
   int rnext0;       // receiver chains
   int nrec;
-  int ring0;        // FP16 scales: ring s (8 words), ring r (8), max |model| (1)
-  float gain;       // FP16 bound gain: 2 + max(V) * tapsum
+  int ring0;        // scaled storage: the shot's kRingWords words of rings
+  float gain;       // bound gain of scaled storage: 2 + max(V) * tapsum
+  float vmax;       // max(V) of the shot
+  int per_cell;     // most receivers sharing one cell
 };
-constexpr int kRingWords = 32;  // ring words per shot
+// Ring words of a shot (scaled storage), from ShotDesc::ring0: the s ring
+// (scales, maxima), the r ring, max |model|, the adjoint state's ring and
+// max |trace|.
+constexpr int kRingWords = 32;
+enum { BR_S = 0, BR_R = 8, BR_MAXM = 16, BR_U = 20, BR_MAXTR = 28 };
 
-// One step of a batch: a.* hold the arrays of the whole batch (shot 0 at
-// offset 0), moved per shot by the descriptors.
-struct BatchFwdStep {
-  FwdStep a;  // a.amp: amplitudes [step][npts_total], shot s at column pts0
+// What every batched step shares: the shot placement and per-shot tables.
+// The step's own arrays hold the whole batch (shot 0 at offset 0) and are
+// moved per shot by the descriptors.
+struct BatchCommon {
   int npts_total;
   const float *sponge;
   float *rings;
   const ShotDesc *desc;      // device copy
   const ShotDesc *host_desc; // host copy (tile counts)
   int nshots;
-  int step;
+  int step;                  // amplitude row of the step
+};
+struct BatchFwdStep : BatchCommon {
+  FwdStep a;  // a.amp: amplitudes [step][npts_total], shot s at column pts0
+};
+// background recompute of the batched exact adjoint; a.hscale (scaled
+// storage): this level's scales, one per shot
+struct BatchSrecStep : BatchCommon {
+  SrecStep a;
+};
+// adjoint-state step of the batched exact adjoint; a.inj: sum |tab_d| of
+// the receiver table row set (times each shot's max(V) and receivers per
+// cell on the device); a.hscale: the sigma level's scales, one per shot
+struct BatchLamStep : BatchCommon {
+  LamStep a;
 };
 
 hipError_t launch_fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a,
@@ -191,6 +211,14 @@ hipError_t launch_lam_pair_step(const Geometry &geo, const Tune &tu, const LamPa
 hipError_t launch_batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
                                  hipStream_t st);
 hipError_t launch_batch_ring_init(float *rings, int nshots, hipStream_t st);
+hipError_t launch_batch_srec_step(const Taps &taps, const Tune &tu, const BatchSrecStep &b,
+                                  hipStream_t st);
+hipError_t launch_batch_lam_step(const Taps &taps, const Tune &tu, const BatchLamStep &b,
+                                 hipStream_t st);
+// ring words [at, at + 4) = 1 (scales) and [at + 4, at + 8) = 0 (maxima) of
+// every shot; with maxima_only, one word at `at` = 0
+hipError_t launch_batch_ring_reset(float *rings, int nshots, int at, int maxima_only,
+                                   hipStream_t st);
 hipError_t launch_src_step(const Geometry &geo, const Tune &tu, const SrcStep &a,
                            hipStream_t st);
 hipError_t launch_radj_step(const Geometry &geo, const Tune &tu, const RadjStep &a,

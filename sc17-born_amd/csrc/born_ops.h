@@ -42,11 +42,18 @@ __device__ __forceinline__ int rec_first(const RecMap &rm, int slot, int x, int 
   return rm.rmap[((long long)slot * ny + y) * nx + x];
 }
 
-// FP16 storage state shared by the ops (unused with float storage): the
-// bound on |new level| is gain * max|cur| + max|old| + injections, with
-// gain = 2 + max(V) * tapsum and tapsum >= |L u| / max|u|.
+// Scaled storage (FP16, or FP16 with a refinement byte) shares the state of
+// the ops (unused with float storage): each stored level has a power-of-two
+// scale derived from a bound on |new level|, gain * max|cur| + max|old| +
+// injections, with gain = 2 + max(V) * tapsum and tapsum >= |L u| / max|u|.
 template <class T>
-constexpr bool kHalf = std::is_same<T, __half>::value;
+constexpr bool kScaled = std::is_same<T, __half>::value || std::is_same<T, Split>::value;
+// storage types with one cell per lane only (VX = 1)
+template <class T>
+constexpr bool kSplit = std::is_same<T, Split>::value;
+// two fields stored together: FP16 as __half2, 24-bit as SplitPair
+template <class T>
+using PairOf = typename std::conditional<kSplit<T>, SplitPair, __half2>::type;
 
 struct Blk1 {  // one stored field
   uint32_t pm;
@@ -68,7 +75,8 @@ __device__ __forceinline__ float amp_sum(const Points &pts, const float *amp) {
 // batch runs without it, see BatchFwd)
 template <int SCATTER, class T, bool NT = true>
 struct FwdOp {
-  static constexpr bool H = kHalf<T>;
+  static constexpr bool H = kScaled<T>;
+  static constexpr bool ONE_CELL = kSplit<T>;  // one cell per lane (VX = 1)
   static constexpr int NF = 2, NPRE = 4, NOUT = 3, NMAX = H ? 2 : 0;
   // FP16 levels hold s and r as one __half2 per cell (sr1: level k, sr0:
   // level k-1, srn: level k+1, the same buffer as sr0 in place)
@@ -83,9 +91,9 @@ struct FwdOp {
   const T *s0;
   T *s_new;
   T *__restrict__ r0;
-  const __half2 *sr1;
-  const __half2 *sr0;
-  __half2 *srn;
+  const PairOf<T> *sr1;
+  const PairOf<T> *sr0;
+  PairOf<T> *srn;
   const float *__restrict__ vel;
   const float *__restrict__ model;
   T *__restrict__ hist;  // optional scatter-operand history
@@ -95,6 +103,7 @@ struct FwdOp {
   float *__restrict__ tr;  // traces, time-major [ntb][nrec]
   const float *__restrict__ tab;
   int id, extract, nx, ny;
+  uint32_t nlev;  // cells of a level (24-bit storage: the lo bytes follow the hi values)
   // FP16 scale state in as few scalar registers as possible (the loop needs
   // none of it, but kernel arguments stay live in scalar registers, and
   // too many of them spill into vector lanes that every use reloads): the
@@ -143,11 +152,11 @@ struct FwdOp {
   }
   template <int VX>
   __device__ void ldv(const Blk2 &b, int f, uint32_t o, float (&v)[VX]) const {
-    ld_vec<VX>(f == 0 ? s1 : r1, o, b.ic[f], v);
+    ld_vec<VX>(f == 0 ? s1 : r1, o, b.ic[f], v, nlev);
   }
   template <int VX>
   __device__ void ldv2(const Blk2 &b, uint32_t o, float (&v)[2][VX]) const {
-    ld_pair<VX>(sr1, o, b.ic, v);
+    ld_pair<VX>(sr1, o, b.ic, v, nlev);
   }
   __device__ SrcPln plane(const Blk2 &b, int z) const {
     return {points_on_plane(pts, b.pm, z), ldc(rm.plane_slot, z)};
@@ -156,18 +165,18 @@ struct FwdOp {
   __device__ void prefetchv(const Blk2 &b, float (&p)[NPRE][VX], uint32_t o) const {
     if constexpr (H) {
       float sr[2][VX];
-      ld_pair<VX>(sr0, o, b.io, sr);
+      ld_pair<VX>(sr0, o, b.io, sr, nlev);
 #pragma unroll
       for (int v = 0; v < VX; v++) {
         p[0][v] = sr[0][v];
         p[1][v] = sr[1][v];
       }
     } else if constexpr (NT) {
-      ld_stream<VX>(s0, o, b.io[0], p[0]);
-      ld_stream<VX>(r0, o, b.io[1], p[1]);
+      ld_stream<VX>(s0, o, b.io[0], p[0], nlev);
+      ld_stream<VX>(r0, o, b.io[1], p[1], nlev);
     } else {
-      ld_vec<VX>(s0, o, b.io[0], p[0]);
-      ld_vec<VX>(r0, o, b.io[1], p[1]);
+      ld_vec<VX>(s0, o, b.io[0], p[0], nlev);
+      ld_vec<VX>(r0, o, b.io[1], p[1], nlev);
     }
     if constexpr (NT) {
       ld_stream<VX>(vel, o, 1.f, p[2]);
@@ -180,12 +189,12 @@ struct FwdOp {
   template <int VX>
   __device__ void storev(const Blk2 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
     if constexpr (H) {
-      st_pair<VX, NT>(srn, o, out[0], out[1], b.sn);
+      st_pair<VX, NT>(srn, o, out[0], out[1], b.sn, nlev);
     } else {
-      st_vec<VX, NT>(s_new, o, out[0], b.sn[0]);
-      st_vec<VX, NT>(r0, o, out[1], b.sn[1]);
+      st_vec<VX, NT>(s_new, o, out[0], b.sn[0], nlev);
+      st_vec<VX, NT>(r0, o, out[1], b.sn[1], nlev);
     }
-    if (hist) st_vec<VX, NT>(hist, o, out[2], b.sh);
+    if (hist) st_vec<VX, NT>(hist, o, out[2], b.sh, nlev);
   }
   __device__ void cell(const Blk2 &b, const SrcPln &pl, const CellArgs &c,
                        const float *lap, const float *ctr, const float *p,
@@ -222,6 +231,7 @@ struct FwdOp {
 // ------------------------------------------------------------- sourceProp
 struct SrcOp {
   static constexpr int NF = 1, NPRE = 2, NOUT = 1, NMAX = 0;
+  static constexpr bool ONE_CELL = false;
   static constexpr bool PAIRED = false;
   static constexpr bool Q2 = false;
   static constexpr bool DAMP = false;
@@ -261,6 +271,7 @@ struct SrcOp {
 // ------------------------------------------------------ reference adjoint
 struct RadjOp {
   static constexpr int NF = 2, NPRE = 5, NOUT = 3, NMAX = 0;
+  static constexpr bool ONE_CELL = false;
   static constexpr bool PAIRED = false;
   static constexpr bool Q2 = false;
   static constexpr bool DAMP = true;
@@ -333,7 +344,8 @@ struct RadjOp {
 // ------------------------------------- exact adjoint: background recompute
 template <int SCATTER, class T>
 struct SrecOp {
-  static constexpr bool H = kHalf<T>;
+  static constexpr bool H = kScaled<T>;
+  static constexpr bool ONE_CELL = kSplit<T>;  // one cell per lane (VX = 1)
   static constexpr int NF = 1, NPRE = 2, NOUT = 2, NMAX = H ? 1 : 0;
   static constexpr bool PAIRED = false;
   static constexpr bool Q2 = H;  // steps in pairs (march_tile)
@@ -352,6 +364,7 @@ struct SrecOp {
   Slots sl;
   float gain, tapsum;
   float *hscale;  // FP16: scale of this step's sigma
+  uint32_t nlev;  // cells of a level
 
   __device__ Blk1 block(int x0, int x1, int y0, int y1, int z0, int z1) const {
     Blk1 b;
@@ -377,14 +390,14 @@ struct SrecOp {
   }
   template <int VX>
   __device__ void ldv(const Blk1 &b, int, uint32_t o, float (&v)[VX]) const {
-    ld_vec<VX>(s1, o, b.ic, v);
+    ld_vec<VX>(s1, o, b.ic, v, nlev);
   }
   __device__ uint32_t plane(const Blk1 &b, int z) const {
     return points_on_plane(pts, b.pm, z);
   }
   template <int VX>
   __device__ void prefetchv(const Blk1 &b, float (&p)[NPRE][VX], uint32_t o) const {
-    ld_stream<VX>(s0, o, b.io, p[0]);
+    ld_stream<VX>(s0, o, b.io, p[0], nlev);
     // FP32: the velocity without the hint (114 -> 103 us per step at 304^3)
     if constexpr (H)
       ld_stream<VX>(vel, o, 1.f, p[1]);
@@ -393,8 +406,8 @@ struct SrecOp {
   }
   template <int VX>
   __device__ void storev(const Blk1 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
-    st_vec<VX>(s_new, o, out[0], b.sn);
-    if (sig) st_vec<VX>(sig, o, out[1], b.sh);
+    st_vec<VX>(s_new, o, out[0], b.sn, nlev);
+    if (sig) st_vec<VX>(sig, o, out[1], b.sh, nlev);
   }
   __device__ void cell(const Blk1 &, uint32_t pm, const CellArgs &c, const float *lap,
                        const float *ctr, const float *p, float (&out)[NOUT],
@@ -413,7 +426,8 @@ struct SrecOp {
 // injection; the image accumulates sigma * nu and is divided by V B at the end.
 template <class T>
 struct LamOp {
-  static constexpr bool H = kHalf<T>;
+  static constexpr bool H = kScaled<T>;
+  static constexpr bool ONE_CELL = kSplit<T>;  // one cell per lane (VX = 1)
   static constexpr int NF = 1, NPRE = 4, NOUT = 2, NMAX = H ? 1 : 0;
   static constexpr bool PAIRED = false;
   static constexpr bool Q2 = H;  // steps in pairs (march_tile)
@@ -433,6 +447,7 @@ struct LamOp {
   float inj;                // bound of |V B P^T w| per unit of max |trace|
   const unsigned *trmax;    // max |trace| as float bits
   const float *hscale;      // FP16: scale of sigma
+  uint32_t nlev;            // cells of a level
 
   __device__ Blk1 block(int, int, int, int, int, int) const {
     Blk1 b;
@@ -457,13 +472,13 @@ struct LamOp {
   }
   template <int VX>
   __device__ void ldv(const Blk1 &b, int, uint32_t o, float (&v)[VX]) const {
-    ld_vec<VX>(u1, o, b.ic, v);
+    ld_vec<VX>(u1, o, b.ic, v, nlev);
   }
   __device__ int plane(const Blk1 &, int z) const { return ldc(rm.plane_slot, z); }
   template <int VX>
   __device__ void prefetchv(const Blk1 &b, float (&p)[NPRE][VX], uint32_t o) const {
-    ld_stream<VX>(u0, o, b.io, p[0]);
-    ld_stream<VX>(sig, o, b.sh, p[2]);
+    ld_stream<VX>(u0, o, b.io, p[0], nlev);
+    ld_stream<VX>(sig, o, b.sh, p[2], nlev);
     // FP32: velocity and image without the non-temporal hint (read, and the
     // image written, every step: 143 against 155 us per step at 304^3);
     // the FP16 step is faster with it
@@ -477,7 +492,7 @@ struct LamOp {
   }
   template <int VX>
   __device__ void storev(const Blk1 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
-    st_vec<VX>(u0, o, out[0], b.sn);
+    st_vec<VX>(u0, o, out[0], b.sn, nlev);
     st_vec<VX, H>(acc, o, out[1], 1.f);
   }
   __device__ void cell(const Blk1 &, int slot, const CellArgs &c, const float *lap,
@@ -542,6 +557,93 @@ struct BatchFwd {
     o.ny = d.ny;
     o.rg = rings + d.ring0;
     o.gain = d.gain;
+    o.nlev = (uint32_t)d.nx * d.ny * d.nz;
+    return r;
+  }
+};
+
+// The exact adjoint's ops for every shot of a batch, moved per shot as
+// BatchFwd: the background recompute and the adjoint-state step, with each
+// shot's scale rings, bound gain, receiver tables and traces.
+template <int SCATTER, class T>
+struct BatchSrec {
+  using Op = SrecOp<SCATTER, T>;
+  Op base;
+  const float *sponge;
+  float *rings;
+  const ShotDesc *__restrict__ desc;
+  int step, npts_total;
+  struct Shot {
+    Grid g;
+    Sponge sp;
+    Op op;
+  };
+  __device__ Shot shot(int s, int tw, int ty, int zc) const {
+    const ShotDesc d = desc[s];
+    Shot r;
+    r.g = tile_grid(d.nx, d.ny, d.nz, tw, ty, zc);
+    r.sp.bx = sponge + d.sp0;
+    r.sp.by = sponge + d.sp0 + d.nx;
+    r.sp.bz = sponge + d.sp0 + d.nx + d.ny;
+    Op &o = r.op;
+    o = base;
+    o.s1 = base.s1 + d.cell0;
+    o.s0 = base.s0 + d.cell0;
+    o.s_new = base.s_new + d.cell0;
+    o.vel = base.vel + d.cell0;
+    o.sig = base.sig ? base.sig + d.cell0 : nullptr;
+    o.pts.p = base.pts.p + d.pts0;
+    o.pts.n = d.npts;
+    o.amp = base.amp + (long long)step * npts_total + d.pts0;
+    o.ring = {rings + d.ring0 + BR_S, (unsigned *)(rings + d.ring0 + BR_S + 4)};
+    o.gain = d.gain;
+    o.hscale = base.hscale ? base.hscale + s : nullptr;
+    o.nlev = (uint32_t)d.nx * d.ny * d.nz;
+    return r;
+  }
+};
+
+template <class T>
+struct BatchLam {
+  using Op = LamOp<T>;
+  Op base;
+  const float *sponge;
+  float *rings;
+  const ShotDesc *__restrict__ desc;
+  int step, npts_total;
+  struct Shot {
+    Grid g;
+    Sponge sp;
+    Op op;
+  };
+  __device__ Shot shot(int s, int tw, int ty, int zc) const {
+    const ShotDesc d = desc[s];
+    Shot r;
+    r.g = tile_grid(d.nx, d.ny, d.nz, tw, ty, zc);
+    r.sp.bx = sponge + d.sp0;
+    r.sp.by = sponge + d.sp0 + d.nx;
+    r.sp.bz = sponge + d.sp0 + d.nx + d.ny;
+    Op &o = r.op;
+    o = base;
+    o.u1 = base.u1 + d.cell0;
+    o.u0 = base.u0 + d.cell0;
+    o.vel = base.vel + d.cell0;
+    o.sig = base.sig + d.cell0;
+    o.acc = base.acc + d.cell0;
+    o.rm.plane_slot = base.rm.plane_slot + d.slot0;
+    o.rm.rmap = base.rm.rmap + d.rmap0;
+    o.rm.rnext = base.rm.rnext + d.rnext0;
+    o.rm.nrec = d.nrec;
+    o.tr = base.tr + d.tr0;
+    o.nx = d.nx;
+    o.ny = d.ny;
+    o.ring = {rings + d.ring0 + BR_U, (unsigned *)(rings + d.ring0 + BR_U + 4)};
+    o.gain = d.gain;
+    // the single-shot bound's factors in its order: (max(V) |tab|) per cell
+    o.inj = d.vmax * base.inj * (float)d.per_cell;
+    o.trmax = (const unsigned *)(rings + d.ring0 + BR_MAXTR);
+    o.hscale = base.hscale ? base.hscale + s : nullptr;
+    o.nlev = (uint32_t)d.nx * d.ny * d.nz;
     return r;
   }
 };
@@ -584,12 +686,16 @@ inline Grid make_grid(const Geometry &geo, const TileShape &ts, int zc) {
 
 template <class Op, int RY, int WY, int VX, int LA, int LX>
 hipError_t launch_shape(const Geometry &geo, int zc, const Op &op, hipStream_t st) {
-  if (VX == 2 && geo.nx % 2) return hipErrorInvalidValue;
-  const Grid g = make_grid(geo, {RY, WY, VX, LA, LX}, zc);
-  const int nb = g.ntx * g.nty * g.ntz;
-  hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX, LA, LX>), dim3(nb), dim3(kTX * WY), 0, st,
-                     g, geo.taps, geo.sponge, op);
-  return hipGetLastError();
+  if constexpr (VX == 2 && Op::ONE_CELL) {
+    return hipErrorInvalidValue;
+  } else {
+    if (VX == 2 && geo.nx % 2) return hipErrorInvalidValue;
+    const Grid g = make_grid(geo, {RY, WY, VX, LA, LX}, zc);
+    const int nb = g.ntx * g.nty * g.ntz;
+    hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX, LA, LX>), dim3(nb), dim3(kTX * WY), 0, st,
+                       g, geo.taps, geo.sponge, op);
+    return hipGetLastError();
+  }
 }
 
 template <class Op>
@@ -609,8 +715,9 @@ T *as(const void *p) {
 }
 
 template <class BOp, int RY, int WY, int VX, int LA, int LX>
-hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdStep &b,
+hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchCommon &b,
                                      const BOp &op, hipStream_t st) {
+  if constexpr (VX == 2 && BOp::Op::ONE_CELL) return hipErrorInvalidValue;
   if (b.nshots < 1 || b.nshots > kMaxBatch) return hipErrorInvalidValue;
   BatchTiles bt{};
   bt.n = b.nshots;
@@ -624,13 +731,14 @@ hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchFwdStep &b,
     tiles += g.ntx * g.nty * g.ntz;
   }
   bt.tile0[b.nshots] = tiles;
-  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX, LA, LX>), dim3(tiles),
-                     dim3(kTX * WY), 0, st, bt, taps, op);
+  if constexpr (!(VX == 2 && BOp::Op::ONE_CELL))
+    hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX, LA, LX>), dim3(tiles),
+                       dim3(kTX * WY), 0, st, bt, taps, op);
   return hipGetLastError();
 }
 
 template <class BOp>
-hipError_t launch_batch(const Taps &taps, const Tune &tu, const BatchFwdStep &b,
+hipError_t launch_batch(const Taps &taps, const Tune &tu, const BatchCommon &b,
                                const BOp &op, hipStream_t st) {
   switch (tu.shape) {
     BORN_SHAPE_CASES(launch_batch_shape, BOp, taps, tu.zchunk, b, op, st)
@@ -647,14 +755,15 @@ hipError_t fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a, hipSt
   FwdOp<SCATTER, T> op{};
   op.s1 = as<T>(a.s1); op.r1 = as<T>(a.r1); op.s0 = as<T>(a.s0); op.r0 = as<T>(a.r0);
   op.s_new = as<T>(a.s_new ? a.s_new : a.s0);
-  op.sr1 = (const __half2 *)a.s1; op.sr0 = (const __half2 *)a.s0;
-  op.srn = (__half2 *)(a.s_new ? a.s_new : a.s0);
+  op.sr1 = (const PairOf<T> *)a.s1; op.sr0 = (const PairOf<T> *)a.s0;
+  op.srn = (PairOf<T> *)(a.s_new ? a.s_new : a.s0);
   op.vel = a.vel; op.model = a.model; op.hist = as<T>(a.hist);
   op.pts = to_points(a.src); op.amp = a.amp;
   op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
   op.extract = a.extract; op.nx = geo.nx; op.ny = geo.ny;
   op.rg = a.rg; op.slots = pack_slots(a.h.sl);
   op.gain = a.h.gain; op.tapsum = a.h.tapsum; op.hscale = a.hscale;
+  op.nlev = (uint32_t)geo.nx * geo.ny * geo.nz;
   return launch(geo, tu, op, st);
 }
 
@@ -666,6 +775,7 @@ hipError_t srec_step(const Geometry &geo, const Tune &tu, const SrecStep &a, hip
   op.pts = to_points(a.src); op.amp = a.amp;
   op.ring = a.ring; op.sl = a.h.sl; op.gain = a.h.gain; op.tapsum = a.h.tapsum;
   op.hscale = a.hscale;
+  op.nlev = (uint32_t)geo.nx * geo.ny * geo.nz;
   return launch(geo, tu, op, st);
 }
 
@@ -677,6 +787,7 @@ hipError_t lam_step(const Geometry &geo, const Tune &tu, const LamStep &a, hipSt
   op.inject = a.inject; op.nx = geo.nx; op.ny = geo.ny;
   op.ring = a.ring; op.sl = a.h.sl; op.gain = a.h.gain; op.inj = a.inj;
   op.trmax = a.trmax; op.hscale = a.hscale;
+  op.nlev = (uint32_t)geo.nx * geo.ny * geo.nz;
   return launch(geo, tu, op, st);
 }
 
@@ -688,12 +799,42 @@ hipError_t batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFwdStep &
   auto &op = bop.base;
   op.s1 = as<T>(a.s1); op.r1 = as<T>(a.r1); op.s0 = as<T>(a.s0); op.r0 = as<T>(a.r0);
   op.s_new = as<T>(a.s0);
-  op.sr1 = (const __half2 *)a.s1; op.sr0 = (const __half2 *)a.s0; op.srn = (__half2 *)a.s0;
+  op.sr1 = (const PairOf<T> *)a.s1; op.sr0 = (const PairOf<T> *)a.s0;
+  op.srn = (PairOf<T> *)a.s0;
   op.vel = a.vel; op.model = a.model; op.hist = nullptr;
   op.pts = to_points(a.src); op.amp = a.amp;
   op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
   op.extract = a.extract;
   op.slots = pack_slots(a.h.sl); op.tapsum = a.h.tapsum; op.hscale = nullptr;
+  bop.sponge = b.sponge; bop.rings = b.rings; bop.desc = b.desc; bop.step = b.step;
+  bop.npts_total = b.npts_total;
+  return launch_batch(taps, tu, b, bop, st);
+}
+
+template <int SCATTER, class T>
+hipError_t batch_srec_step(const Taps &taps, const Tune &tu, const BatchSrecStep &b,
+                           hipStream_t st) {
+  const SrecStep &a = b.a;
+  BatchSrec<SCATTER, T> bop{};
+  auto &op = bop.base;
+  op.s1 = as<T>(a.s1); op.s0 = as<T>(a.s0); op.vel = a.vel; op.sig = as<T>(a.sig);
+  op.s_new = as<T>(a.s_new ? a.s_new : a.s0);
+  op.pts = to_points(a.src); op.amp = a.amp;
+  op.sl = a.h.sl; op.tapsum = a.h.tapsum; op.hscale = a.hscale;
+  bop.sponge = b.sponge; bop.rings = b.rings; bop.desc = b.desc; bop.step = b.step;
+  bop.npts_total = b.npts_total;
+  return launch_batch(taps, tu, b, bop, st);
+}
+
+template <class T>
+hipError_t batch_lam_step(const Taps &taps, const Tune &tu, const BatchLamStep &b,
+                          hipStream_t st) {
+  const LamStep &a = b.a;
+  BatchLam<T> bop{};
+  auto &op = bop.base;
+  op.u1 = as<T>(a.u1); op.u0 = as<T>(a.u0); op.vel = a.vel; op.sig = as<T>(a.sig);
+  op.acc = a.acc; op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
+  op.inject = a.inject; op.sl = a.h.sl; op.inj = a.inj; op.hscale = a.hscale;
   bop.sponge = b.sponge; bop.rings = b.rings; bop.desc = b.desc; bop.step = b.step;
   bop.npts_total = b.npts_total;
   return launch_batch(taps, tu, b, bop, st);
@@ -706,6 +847,11 @@ hipError_t batch_fwd_step(const Taps &taps, const Tune &tu, const BatchFwdStep &
   F hipError_t srec_step<S, T>(const Geometry &, const Tune &, const SrecStep &, hipStream_t)
 #define BORN_LAM_STEP(F, T) \
   F hipError_t lam_step<T>(const Geometry &, const Tune &, const LamStep &, hipStream_t)
+#define BORN_BATCH_SREC_STEP(F, S, T)                                                   \
+  F hipError_t batch_srec_step<S, T>(const Taps &, const Tune &, const BatchSrecStep &, \
+                                     hipStream_t)
+#define BORN_BATCH_LAM_STEP(F, T) \
+  F hipError_t batch_lam_step<T>(const Taps &, const Tune &, const BatchLamStep &, hipStream_t)
 #define BORN_BATCH_FWD_STEP(F, S, T)                                                     \
   F hipError_t batch_fwd_step<S, T>(const Taps &, const Tune &, const BatchFwdStep &, \
                                     hipStream_t)

@@ -69,6 +69,10 @@ SIGNATURES = {
     "born_batch_destroy": (_i, [_vp]),
     "born_batch_forward": (_i, [_vp, _i, _i, _i]),
     "born_batch_traces": (_i, [_vp, _i, _vp]),
+    "born_batch_adjoint_exact": (_i, [_vp, _i, _i, _i]),
+    "born_batch_set_traces": (_i, [_vp, _i, _vp]),
+    "born_batch_image": (_i, [_vp, _i, _vp]),
+    "born_batch_last_checkpoint_plan": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "born_batch_get_tile": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
                                  C.POINTER(C.c_float)]),
     "born_synchronize": (_i, [_vp]),
@@ -234,12 +238,14 @@ class Propagator:
     KERNELS = ("fwd", "fwd_dV", "source", "rtm_adjoint", "background", "background_dV",
                "adjoint_state", "fwd_f16", "fwd_dV_f16", "background_f16",
                "background_dV_f16", "adjoint_state_f16", "fwd_pair", "fwd_dV_pair",
-               "adjoint_state_pair")
-    STORAGE = {"f32": 0, "f16": 1}
+               "adjoint_state_pair", "fwd_f24", "fwd_dV_f24", "background_f24",
+               "background_dV_f24", "adjoint_state_f24")
+    STORAGE = {"f32": 0, "f16": 1, "f24": 2}
 
     def set_storage(self, kind):
-        """Wavefield storage of the Born forward / exact adjoint: "f32" or
-        "f16" (FP16 levels with per-level power-of-two scales)."""
+        """Wavefield storage of the Born forward / exact adjoint: "f32",
+        "f16" (FP16 levels with per-level power-of-two scales) or "f24" (the
+        same scaled FP16 plus a refinement byte per cell)."""
         self._chk(self.lib.born_set_storage(self.h, self.STORAGE[kind]), "storage")
 
     def set_step_pairs(self, mode):
@@ -451,7 +457,7 @@ class Batch:
         self.lib = prop.lib
         self._keep = []
         arr = (_Shot * len(shots))()
-        self.nrec, self.ntb = [], []
+        self.nrec, self.ntb, self.cells = [], [], []
         for k, sh in enumerate(shots):
             vel = np.ascontiguousarray(sh["vel"], np.float32)
             model = np.ascontiguousarray(sh["model"], np.float32)
@@ -463,6 +469,7 @@ class Batch:
                            sv.shape[-1], _ptr(sl), _ptr(sv), sh["rec_nx"], sh["rec_ny"],
                            _ptr(rl))
             self.nrec.append(sh["rec_nx"] * sh["rec_ny"])
+            self.cells.append(sh["nx"] * sh["ny"] * sh["nz"])
             self.ntb.append(sv.shape[-1])
         h = _vp()
         d1, d2, d3 = d
@@ -489,6 +496,27 @@ class Batch:
         out = np.zeros((self.nrec[shot], self.ntb[shot]), np.float32)
         self.prop._chk(self.lib.born_batch_traces(self.h, shot, _ptr(out)), "batch_traces")
         return out
+
+    def adjoint_exact(self, jt, nt, scatter=0):
+        """images of every shot from the current traces (born_batch_adjoint_exact)"""
+        self.prop._chk(self.lib.born_batch_adjoint_exact(self.h, jt, nt, scatter),
+                       "batch_adjoint_exact")
+
+    def set_traces(self, shot, rec):
+        r = np.ascontiguousarray(rec, np.float32)
+        assert r.shape == (self.nrec[shot], self.ntb[shot])
+        self.prop._chk(self.lib.born_batch_set_traces(self.h, shot, _ptr(r)), "batch_set_traces")
+
+    def image(self, shot):
+        out = np.zeros(self.cells[shot], np.float32)
+        self.prop._chk(self.lib.born_batch_image(self.h, shot, _ptr(out)), "batch_image")
+        return out
+
+    def last_checkpoint_plan(self):
+        s, r = C.c_int(), C.c_int()
+        self.prop._chk(self.lib.born_batch_last_checkpoint_plan(self.h, C.byref(s), C.byref(r)),
+                       "batch_plan")
+        return s.value, r.value
 
     def tile(self, scatter=0):
         tx, ty, zc, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()

@@ -104,3 +104,73 @@ def test_batch_argument_errors(pyborn):
     with pytest.raises(pyborn.BornError, match="UNSUPPORTED"):
         pyborn.Batch(H, [shots[0], bad], cases[0].d)
     H.close()
+
+
+def _single_pair(pyborn, c, storage, scatter):
+    """single-shot forward, then the exact adjoint of its own traces"""
+    H = hip_prop(pyborn, c)
+    H.set_storage(storage)
+    rec = np.zeros((c.nrec, c.ntb), np.float32)
+    H.transfer_receiver_func(c.rec_nx, c.rec_ny, c.rec_locs, rec)
+    H.forward_scatter(c.jt, c.model, rec, c.nt, scatter)
+    img = np.zeros(c.nx * c.ny * c.nz, np.float32)
+    H.adjoint_exact(c.jt, rec, img, c.nt, scatter)
+    H.close()
+    return rec, img
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "f24"])
+@pytest.mark.parametrize("scatter", [0, 1])
+def test_batch_pair_matches_single_shot_pairs(pyborn, storage, scatter):
+    """born_batch_forward then born_batch_adjoint_exact: every shot's image
+    is the single-shot exact adjoint of its traces, bit for bit"""
+    cases = _cases(SHAPES)
+    H, B = _batch(pyborn, cases, storage)
+    B.forward(1, cases[0].nt, scatter)
+    B.adjoint_exact(1, cases[0].nt, scatter)
+    imgs = [B.image(s) for s in range(len(cases))]
+    assert B.last_checkpoint_plan() == (1, 0)
+    B.close()
+    H.close()
+    for c, g in zip(cases, imgs):
+        _, want = _single_pair(pyborn, c, storage, scatter)
+        assert np.abs(want).max() > 0
+        print("%s scatter %d shot %dx%dx%d equal %s" % (storage, scatter, c.nx, c.ny, c.nz,
+                                                        np.array_equal(g, want)))
+        assert np.array_equal(g, want)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f24"])
+def test_batch_adjoint_checkpoint_plans_agree(pyborn, storage):
+    cases = _cases(SHAPES, nt=70)
+    H, B = _batch(pyborn, cases, storage)
+    rng = np.random.default_rng(8)
+    for s, c in enumerate(cases):
+        B.set_traces(s, rng.standard_normal((c.nrec, c.ntb)).astype(np.float32))
+    B.adjoint_exact(1, 70, 0)
+    full = [B.image(s) for s in range(len(cases))]
+    assert B.last_checkpoint_plan() == (1, 0)
+    esz = {"f32": 4, "f24": 3}[storage]
+    # the batch's level: every shot's cells, each shot padded to 64 cells
+    level = sum(-(-c.nx * c.ny * c.nz // 64) * 64 for c in cases) * esz
+    for budget_levels, expect in [(40, 2), (25, 4)]:
+        H.set_checkpoint_budget(budget_levels * level)
+        B.adjoint_exact(1, 70, 0)
+        segs, rec = B.last_checkpoint_plan()
+        L = -(-70 // segs)
+        assert segs == expect and rec == (segs - 1) * L
+        for s in range(len(cases)):
+            assert np.array_equal(B.image(s), full[s])
+    B.close()
+    H.close()
+    # the set traces give the single-shot exact adjoint of those traces
+    rng = np.random.default_rng(8)
+    for s, c in enumerate(cases):
+        w = rng.standard_normal((c.nrec, c.ntb)).astype(np.float32)
+        S = hip_prop(pyborn, c)
+        S.set_storage(storage)
+        S.transfer_receiver_func(c.rec_nx, c.rec_ny, c.rec_locs, np.zeros_like(w))
+        img = np.zeros(c.nx * c.ny * c.nz, np.float32)
+        S.adjoint_exact(1, w, img, 70, 0)
+        S.close()
+        assert np.array_equal(full[s], img)

@@ -1,7 +1,8 @@
-"""FP16 storage error on the 512x512x400 configuration's physics, on a 128^3
-crop (the FP64 oracle's share): data, image and gradient of the Born pair
-against the FP64 oracle, 2500 steps at 0.8 ms.  Takes minutes of CPU time
-for the oracle, so it runs only with BORN_CROP_STUDY=1.
+"""16-bit-based storage error on the 512x512x400 configuration's physics, on
+a 128^3 crop (the FP64 oracle's share): data, image and gradient of the Born
+pair against the FP64 oracle, 2500 steps at 0.8 ms, for FP32 levels, FP16
+levels (an approximate mode) and 24-bit levels (FP16 plus a refinement byte,
+held to the 1e-3 bar).  The oracle takes a few minutes of CPU time.
 
   data     F m                    (m: smoothed noise, z-differenced)
   image    F^T d64                (d64: the FP64 data)
@@ -17,18 +18,18 @@ import oracle
 from cases import rel_l2
 from conftest import load_pkg_module
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(not os.environ.get("BORN_CROP_STUDY"),
-                                 reason="long study; set BORN_CROP_STUDY=1")]
+pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.timeout(900)
 def test_fp16_errors_on_a_128_cube_crop(pyborn):
     os.environ["ORC_PROGRESS"] = "1"  # the oracle loops report their steps
     configs = load_pkg_module("configs")
     p = configs.make_problem(**configs.C3_CROP)
     nt = p.nt
     locs, vals = p.shots[0]
-    P = oracle.Prop(0)
+    env = os.environ.get("OMP_NUM_THREADS")
+    P = oracle.Prop(int(env) if env else len(os.sched_getaffinity(0)))
     P.create_space(p.d, p.d, p.d, p.nx, p.ny, p.nz)
     P.transfer_sinc_table_s(p.tab)
     P.transfer_sinc_table_d(p.tab)
@@ -42,7 +43,7 @@ def test_fp16_errors_on_a_128_cube_crop(pyborn):
     d32in = d64.astype(np.float32)
     m0 = (0.5 * p.model).astype(np.float32)
     out = {"domain": [p.nx, p.ny, p.nz], "steps": nt}
-    for storage in ("f32", "f16"):
+    for storage in ("f32", "f16", "f24"):
         H = pyborn.Propagator(0)
         H.create_space(p.d, p.d, p.d, p.nx, p.ny, p.nz)
         H.transfer_sinc_table_s(p.tab)
@@ -67,4 +68,7 @@ def test_fp16_errors_on_a_128_cube_crop(pyborn):
         print(storage, out[storage], flush=True)
     print("crop study", json.dumps(out))
     assert max(out["f32"].values()) <= 1e-4
-    assert max(out["f16"].values()) <= 1e-1
+    # FP16 levels are an approximate mode: rounding noise injected at every
+    # step grows with the step count (measured 9.5e-3 / 4.4e-3 / 6.9e-3)
+    assert max(out["f16"].values()) <= 1.5e-2
+    assert max(out["f24"].values()) <= 1e-3
