@@ -142,6 +142,20 @@ int born_dev_source_prop(born_ctx *ctx, int jts, int nt, float *p0, float *p1);
 int born_dev_rtm_adjoint(born_ctx *ctx, int nt, float *p0, float *p1,
                          float *img, const float *traces);
 
+/* born_dev_source_prop without allocating anything, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.
+ * The context must be set up by the transfer calls first (they allocate);
+ * this call then runs no autotuning (it uses the launch shape tuned on this
+ * context so far, else the default shape: every shape gives the same
+ * values) and keeps its only scratch, the per-step source amplitudes, in a
+ * fixed device array of 2^20 floats (4 MiB) that each GPU holds for the life
+ * of the process.  (nt + 1) x source points above 2^20 returns BORN_ERR_ARG.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (several contexts or host threads) race; calls queued on one stream
+ * do not.  Results equal born_dev_source_prop's. */
+int born_dev_source_prop_noalloc(born_ctx *ctx, int jts, int nt, float *p0, float *p1);
+
 /* HBM budget (bytes) for the background history of the exact adjoint;
  * 0 = all free device memory less a reserve.  The number of checkpoints is
  * the smallest that fits (1 segment = the whole history kept). */

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,11 @@ struct born_ctx {
   int last_segments = 0, last_recomputed = 0;
   int last_pairs = 0;
   int last_lam_pairs = 0;
+
+  // this device's scratch array of born_dev_source_prop_noalloc (resolved
+  // at born_create), or the reason it could not be resolved
+  float *scratch = nullptr;
+  std::string scratch_err;
 };
 
 namespace {
@@ -681,6 +687,43 @@ int check_jt(born_ctx *ctx, int jt) {
 
 }  // namespace
 
+// ---- Scratch of the allocation-free entry point born_dev_source_prop_noalloc:
+// the per-step source amplitudes, at most kScratchFloats of them, in a
+// file-scope device array that each GPU holds for the life of the process.
+// Host code never uses the array's name as an address: its device address
+// on each GPU is resolved once (hipGetSymbolAddress, with that GPU current)
+// and kept per device.  Every call on one GPU shares that GPU's array.
+constexpr size_t kScratchFloats = size_t(1) << 20;  // 4 MiB per GPU
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch[kMaxDevices] = {};
+
+// the scratch address of ctx's device (the device is current), once per device
+void scratch_setup(born_ctx *ctx) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (ctx->device >= kMaxDevices) {
+    ctx->scratch_err = "device index beyond " + std::to_string(kMaxDevices);
+    return;
+  }
+  if (!g_scratch[ctx->device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      ctx->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return;
+    }
+    g_scratch[ctx->device] = (float *)p;
+  }
+  ctx->scratch = g_scratch[ctx->device];
+}
+
+}  // namespace
+
 // ======================================================================
 extern "C" {
 
@@ -702,6 +745,7 @@ int born_create(born_ctx **out, int device) {
     delete ctx;
     return BORN_ERR_HIP;
   }
+  scratch_setup(ctx);
   *out = ctx;
   return BORN_OK;
 }
@@ -1127,6 +1171,45 @@ int born_dev_source_prop(born_ctx *ctx, int jts, int nt, float *p0, float *p1) {
   if (nt % 2 == 1)  // cpu_prop.cc:123-128
     HIPCHK(hipMemcpyAsync(p1, p0, (size_t)ctx->n * sizeof(float),
                           hipMemcpyDeviceToDevice, ctx->stream));
+  return BORN_OK;
+}
+
+// born_dev_source_prop without allocating: no autotuning (the launch shape
+// tuned on this context so far, else the default one; every shape gives the
+// same values) and the amplitudes in this device's scratch array.
+int born_dev_source_prop_noalloc(born_ctx *ctx, int jts, int nt, float *p0, float *p1) {
+  CHK(need_born_inputs(ctx, true, false, false));
+  if (nt < 0 || jts < 1 || !p0 || !p1) return fail(ctx, BORN_ERR_ARG, "bad source_prop arguments");
+  if (!ctx->scratch)
+    return fail(ctx, BORN_ERR_HIP, "no scratch array on this device: " + ctx->scratch_err);
+  const size_t need = (size_t)(nt + 1) * (size_t)ctx->npts_s;
+  if (need > kScratchFloats)
+    return fail(ctx, BORN_ERR_ARG,
+                "(nt + 1) x source points = " + std::to_string(need) +
+                    " amplitudes exceed the scratch array (" + std::to_string(kScratchFloats) +
+                    ")");
+  CHK(set_device(ctx));
+  ctx->jt_state = jts;
+  const Geometry geo = geometry(ctx);
+  const born::Tune tu = tuning(ctx, K_SRC);
+  // every amplitude the loop reads is written here first
+  HIPCHK(born::launch_src_amp(ctx->scratch, nt + 1, ctx->npts_s, ctx->d_src, ctx->nt_src,
+                              ctx->d_tab_s, ctx->nsinc_s, ctx->jts, ctx->jts, 0,
+                              1 / (float)jts, ctx->stream));
+  float *a0 = p0, *a1 = p1;
+  born::SrcStep s{};
+  s.vel = ctx->d_vel1;
+  s.src = points(ctx);
+  for (int it = 0; it <= nt; it++) {
+    s.s1 = a1;
+    s.s0 = a0;
+    s.amp = ctx->scratch + (long long)it * ctx->npts_s;
+    HIPCHK(born::launch_src_step(geo, tu, s, ctx->stream));
+    std::swap(a0, a1);
+  }
+  if (nt % 2 == 1)  // cpu_prop.cc:123-128
+    HIPCHK(hipMemcpyAsync(p1, p0, (size_t)ctx->n * sizeof(float), hipMemcpyDeviceToDevice,
+                          ctx->stream));
   return BORN_OK;
 }
 

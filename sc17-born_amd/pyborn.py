@@ -52,6 +52,7 @@ SIGNATURES = {
     "born_dev_adjoint_exact": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "born_dev_source_prop": (_i, [_vp, _i, _i, _vp, _vp]),
     "born_dev_rtm_adjoint": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "born_dev_source_prop_noalloc": (_i, [_vp, _i, _i, _vp, _vp]),
     "born_set_checkpoint_budget": (_i, [_vp, C.c_size_t]),
     "born_last_checkpoint_plan": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "born_set_zchunk": (_i, [_vp, _i]),
@@ -303,6 +304,12 @@ class Propagator:
     def dev_source_prop(self, jts, nt, p0, p1):
         self._chk(self.lib.born_dev_source_prop(self.h, jts, nt, _dptr(p0), _dptr(p1)),
                   "dev_source_prop")
+
+    def dev_source_prop_noalloc(self, jts, nt, p0, p1):
+        """dev_source_prop without any allocation (scratch in a fixed device
+        array; see born_dev_source_prop_noalloc)"""
+        self._chk(self.lib.born_dev_source_prop_noalloc(self.h, jts, nt, _dptr(p0), _dptr(p1)),
+                  "dev_source_prop_noalloc")
 
     def dev_rtm_adjoint(self, nt, p0, p1, img, traces):
         self._chk(self.lib.born_dev_rtm_adjoint(self.h, nt, _dptr(p0), _dptr(p1), _dptr(img),
