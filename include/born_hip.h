@@ -174,11 +174,11 @@ int born_last_checkpoint_plan(const born_ctx *ctx, int *segments,
 #define BORN_STORAGE_F32 0
 #define BORN_STORAGE_F16 1
 /* BORN_STORAGE_F24: every level as an FP16 value plus a refinement byte per
- * cell (3 bytes), with the same per-level scales as BORN_STORAGE_F16: a level
- * is stored to 2^20 steps of its range instead of FP16's 11 significant bits,
- * so the rounding noise injected at every step no longer dominates the error
- * of thousands of steps.  Kernels take one cell per lane; fields of 24-bit
- * levels hold fewer than 715,827,882 cells. */
+ * cell (3 bytes), with the same per-level scales as BORN_STORAGE_F16: every
+ * value keeps 19 significant bits instead of FP16's 11, so the rounding
+ * noise injected at every step no longer dominates the error
+ * of thousands of steps.  Fields of 24-bit levels hold fewer than
+ * 715,827,882 cells. */
 #define BORN_STORAGE_F24 2
 int born_set_storage(born_ctx *ctx, int storage);
 

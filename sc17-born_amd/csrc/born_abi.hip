@@ -208,8 +208,6 @@ int srec_kind(int storage, int scatter) {
   return k[storage][scatter];
 }
 int lam_kind(int storage) { return storage == 2 ? K_LAMS : storage == 1 ? K_LAMH : K_LAM; }
-// kernels of the 24-bit storage take one cell per lane
-bool one_cell(int kind) { return kind >= K_FWD0S && kind <= K_LAMS; }
 bool is_pair(int kind) { return kind == K_FWD0P || kind == K_FWD1P || kind == K_LAMP; }
 
 born::SrcPts points(const born_ctx *c) { return {c->d_pts, c->npts_s}; }
@@ -316,8 +314,8 @@ int autotune_into(born_ctx *ctx, born::Tune *tune, float *tune_ms, bool even_nx,
 
 template <class F>
 int autotune(born_ctx *ctx, int kind, F run) {
-  return autotune_into(ctx, &ctx->tune[kind], &ctx->tune_ms[kind],
-                       ctx->nx % 2 == 0 && !one_cell(kind), ctx->nz - 2 * born::kH, run);
+  return autotune_into(ctx, &ctx->tune[kind], &ctx->tune_ms[kind], ctx->nx % 2 == 0,
+                       ctx->nz - 2 * born::kH, run);
 }
 
 // The same for a two-step kernel: its shapes and z chunks that split the
@@ -1722,8 +1720,6 @@ int born_set_tuning(born_ctx *ctx, int kernel, int rows_per_thread, int waves_pe
     }
     return fail(ctx, BORN_ERR_ARG, "no such launch shape");
   }
-  if (one_cell(kernel) && cells_per_lane != 1)
-    return fail(ctx, BORN_ERR_ARG, "24-bit storage kernels take one cell per lane");
   for (int sh = 0; sh < born::kNumShapes; sh++) {
     const born::TileShape &ts = born::kShapes[sh];
     if (ts.ry == rows_per_thread && ts.wy == waves_per_block && ts.vx == cells_per_lane &&
@@ -2060,14 +2056,14 @@ int born_batch_forward(born_batch *b, int jt, int nt, int scatter) {
     HIPCHK(born::launch_batch_ring_init(b->d_rings, b->nshots, ctx->stream));
     born::BatchFwdStep t = batch_step(b, 0, jt, scatter, storage);
     t.a.extract = 0;
-    CHK(autotune_into(ctx, &b->tune[kind], &b->tune_ms[kind], b->even_nx && !split, 0,
+    CHK(autotune_into(ctx, &b->tune[kind], &b->tune_ms[kind], b->even_nx, 0,
                       [&](const born::Tune &tu) {
                         return born::launch_batch_fwd_step(taps, tu, t, ctx->stream);
                       }));
   }
   born::Tune tu = b->tune[kind];
   if (ctx->zchunk > 0) tu.zchunk = ctx->zchunk;
-  if (born::kShapes[tu.shape].vx == 2 && (!b->even_nx || split)) tu.shape = 3;
+  if (born::kShapes[tu.shape].vx == 2 && !b->even_nx) tu.shape = 3;
   // initial state: fields and traces zero, amplitudes of every shot, scales
   for (int i = 0; i < 4; i++) CHK(zero(ctx, b->d_f[i], (long long)b->f_cap));
   CHK(zero(ctx, b->d_tr, b->tr_total));
@@ -2201,7 +2197,7 @@ int born_batch_adjoint_exact(born_batch *b, int jt, int nt, int scatter) {
   auto tune_of = [&](int kind) {
     born::Tune t = b->tune[kind];
     if (ctx->zchunk > 0) t.zchunk = ctx->zchunk;
-    if (born::kShapes[t.shape].vx == 2 && (!b->even_nx || split)) t.shape = 3;
+    if (born::kShapes[t.shape].vx == 2 && !b->even_nx) t.shape = 3;
     return t;
   };
   float *s_old = b->d_f[2], *s_cur = b->d_f[3];
@@ -2216,7 +2212,7 @@ int born_batch_adjoint_exact(born_batch *b, int jt, int nt, int scatter) {
     t.a.s1 = s_cur; t.a.s0 = s_old; t.a.vel = b->d_vel; t.a.sig = b->d_acc;
     t.a.src = {b->d_pts, 0}; t.a.amp = b->d_amp; t.a.scatter = scatter;
     t.a.h = hstate(0); t.a.hscale = b->d_hscale;
-    CHK(autotune_into(ctx, &b->tune[skind], &b->tune_ms[skind], b->even_nx && !split, 0,
+    CHK(autotune_into(ctx, &b->tune[skind], &b->tune_ms[skind], b->even_nx, 0,
                       [&](const born::Tune &tu) {
                         return born::launch_batch_srec_step(taps, tu, t, ctx->stream);
                       }));
@@ -2226,7 +2222,7 @@ int born_batch_adjoint_exact(born_batch *b, int jt, int nt, int scatter) {
     l.a.rec.plane_slot = b->d_slot; l.a.rec.rmap = b->d_rmap; l.a.rec.rnext = b->d_rnext;
     l.a.rec.ntb = b->ntb; l.a.tr = b->d_tr; l.a.tab = ctx->d_tab_d; l.a.inject = 0;
     l.a.h = hstate(0); l.a.inj = ctx->tabd_abs; l.a.hscale = b->d_hscale;
-    CHK(autotune_into(ctx, &b->tune[lkind], &b->tune_ms[lkind], b->even_nx && !split, 0,
+    CHK(autotune_into(ctx, &b->tune[lkind], &b->tune_ms[lkind], b->even_nx, 0,
                       [&](const born::Tune &tu) {
                         return born::launch_batch_lam_step(taps, tu, l, ctx->stream);
                       }));

@@ -367,7 +367,7 @@ __device__ __forceinline__ void st_pair(__half2 *p, uint32_t o, const float (&a)
 // hi and lo addresses are a shift of its offset, lo's plus the uniform
 // 2n (4n) bytes.  A pointer to Split (SplitPair) advances by 3 (6) bytes per
 // cell: the levels of a history and the shots of a batch are whole regions
-// of 3n (6n) bytes.  One cell per lane only (VX = 1).
+// of 3n (6n) bytes.
 struct Split {
   unsigned char b[3];
 };
@@ -392,14 +392,22 @@ __device__ __forceinline__ uint32_t f16_f32_bits(unsigned short h) {
   return __float_as_uint(__half2float(__ushort_as_half(h)));
 }
 
+// VX = 2: two consecutive cells per lane (o even, n even: a pair of hi
+// values is 4-byte aligned, its lo bytes 2-byte aligned)
 template <int VX>
 __device__ __forceinline__ void ld_vec(const Split *p, uint32_t o, float inv, float (&v)[VX],
                                        uint32_t n) {
-  static_assert(VX == 1, "24-bit storage: one cell per lane");
   const char *b = (const char *)p;
-  const unsigned short hi = *(const unsigned short *)(b + o * 2u);
-  const uint32_t lo = *(const unsigned char *)(b + (2u * n + o));
-  v[0] = split_val(f16_f32_bits(hi), lo, inv);
+  if constexpr (VX == 1) {
+    const unsigned short hi = *(const unsigned short *)(b + o * 2u);
+    const uint32_t lo = *(const unsigned char *)(b + (2u * n + o));
+    v[0] = split_val(f16_f32_bits(hi), lo, inv);
+  } else {
+    const uint32_t hi = *(const uint32_t *)(b + o * 2u);
+    const uint32_t lo = *(const unsigned short *)(b + (2u * n + o));
+    v[0] = split_val(f16_f32_bits((unsigned short)(hi & 0xffffu)), lo, inv);
+    v[1] = split_val(f16_f32_bits((unsigned short)(hi >> 16)), lo >> 8, inv);
+  }
 }
 template <int VX>
 __device__ __forceinline__ void ld_stream(const Split *p, uint32_t o, float inv,
@@ -409,40 +417,64 @@ __device__ __forceinline__ void ld_stream(const Split *p, uint32_t o, float inv,
 template <int VX, bool NT = true>
 __device__ __forceinline__ void st_vec(Split *p, uint32_t o, const float (&v)[VX], float scale,
                                        uint32_t n) {
-  static_assert(VX == 1, "24-bit storage: one cell per lane");
-  uint32_t lo;
-  const float top = split_top(v[0], scale, lo);
-  // top is FP16-exact where it matters; round toward zero below FP16's range
-  const auto h = __builtin_amdgcn_cvt_pkrtz(top, 0.f);
+  uint32_t l0, l1 = 0;
+  const float t0 = split_top(v[0], scale, l0);
+  // top is FP16-exact where it matters; rounded toward zero below FP16's range
+  float t1 = 0.f;
+  if constexpr (VX == 2) t1 = split_top(v[1], scale, l1);
+  const auto h = __builtin_amdgcn_cvt_pkrtz(t0, t1);
   uint32_t hb;
   __builtin_memcpy(&hb, &h, 4);
-  st_b16<NT>(p, o * 2u, (unsigned short)(hb & 0xffffu));
-  st_b8<NT>(p, 2u * n + o, (unsigned char)lo);
+  if constexpr (VX == 1) {
+    st_b16<NT>(p, o * 2u, (unsigned short)(hb & 0xffffu));
+    st_b8<NT>(p, 2u * n + o, (unsigned char)l0);
+  } else {
+    st_b32<NT>(p, o * 2u, hb);
+    st_b16<NT>(p, 2u * n + o, (unsigned short)(l0 | l1 << 8));
+  }
 }
 // two fields stored together: first field in the low half of each pair
 template <int VX>
 __device__ __forceinline__ void ld_pair(const SplitPair *p, uint32_t o, const float (&inv)[2],
                                         float (&v)[2][VX], uint32_t n) {
-  static_assert(VX == 1, "24-bit storage: one cell per lane");
   const char *b = (const char *)p;
-  const uint32_t hi = *(const uint32_t *)(b + o * 4u);
-  const uint32_t lo = *(const unsigned short *)(b + (4u * n + o * 2u));
-  v[0][0] = split_val(f16_f32_bits((unsigned short)(hi & 0xffffu)), lo, inv[0]);
-  v[1][0] = split_val(f16_f32_bits((unsigned short)(hi >> 16)), lo >> 8, inv[1]);
+  if constexpr (VX == 1) {
+    const uint32_t hi = *(const uint32_t *)(b + o * 4u);
+    const uint32_t lo = *(const unsigned short *)(b + (4u * n + o * 2u));
+    v[0][0] = split_val(f16_f32_bits((unsigned short)(hi & 0xffffu)), lo, inv[0]);
+    v[1][0] = split_val(f16_f32_bits((unsigned short)(hi >> 16)), lo >> 8, inv[1]);
+  } else {
+    const uint2 hi = *(const uint2 *)(b + o * 4u);
+    const uint32_t lo = *(const uint32_t *)(b + (4u * n + o * 2u));
+    v[0][0] = split_val(f16_f32_bits((unsigned short)(hi.x & 0xffffu)), lo, inv[0]);
+    v[1][0] = split_val(f16_f32_bits((unsigned short)(hi.x >> 16)), lo >> 8, inv[1]);
+    v[0][1] = split_val(f16_f32_bits((unsigned short)(hi.y & 0xffffu)), lo >> 16, inv[0]);
+    v[1][1] = split_val(f16_f32_bits((unsigned short)(hi.y >> 16)), lo >> 24, inv[1]);
+  }
 }
 template <int VX, bool NT = true>
 __device__ __forceinline__ void st_pair(SplitPair *p, uint32_t o, const float (&a)[VX],
                                         const float (&b)[VX], const float (&scale)[2],
                                         uint32_t n) {
-  static_assert(VX == 1, "24-bit storage: one cell per lane");
-  uint32_t l0, l1;
-  const float t0 = split_top(a[0], scale[0], l0);
-  const float t1 = split_top(b[0], scale[1], l1);
-  const auto h = __builtin_amdgcn_cvt_pkrtz(t0, t1);
-  uint32_t hb;
-  __builtin_memcpy(&hb, &h, 4);
-  st_b32<NT>(p, o * 4u, hb);
-  st_b16<NT>(p, 4u * n + o * 2u, (unsigned short)(l0 | l1 << 8));
+  uint32_t la0, lb0;
+  const float ta0 = split_top(a[0], scale[0], la0);
+  const float tb0 = split_top(b[0], scale[1], lb0);
+  const auto h0 = __builtin_amdgcn_cvt_pkrtz(ta0, tb0);
+  uint32_t hb0;
+  __builtin_memcpy(&hb0, &h0, 4);
+  if constexpr (VX == 1) {
+    st_b32<NT>(p, o * 4u, hb0);
+    st_b16<NT>(p, 4u * n + o * 2u, (unsigned short)(la0 | lb0 << 8));
+  } else {
+    uint32_t la1, lb1;
+    const float ta1 = split_top(a[1], scale[0], la1);
+    const float tb1 = split_top(b[1], scale[1], lb1);
+    const auto h1 = __builtin_amdgcn_cvt_pkrtz(ta1, tb1);
+    uint32_t hb1;
+    __builtin_memcpy(&hb1, &h1, 4);
+    st_b64<NT>(p, o * 4u, (uint64_t)hb1 << 32 | hb0);
+    st_b32<NT>(p, 4u * n + o * 2u, la0 | lb0 << 8 | la1 << 16 | lb1 << 24);
+  }
 }
 
 // ---- VX consecutive cells of a row (VX = 1 or 2).  A pair starts at an

@@ -48,7 +48,7 @@ __device__ __forceinline__ int rec_first(const RecMap &rm, int slot, int x, int 
 // injections, with gain = 2 + max(V) * tapsum and tapsum >= |L u| / max|u|.
 template <class T>
 constexpr bool kScaled = std::is_same<T, __half>::value || std::is_same<T, Split>::value;
-// storage types with one cell per lane only (VX = 1)
+// 24-bit storage (FP16 with a refinement byte)
 template <class T>
 constexpr bool kSplit = std::is_same<T, Split>::value;
 // two fields stored together: FP16 as __half2, 24-bit as SplitPair
@@ -76,7 +76,6 @@ __device__ __forceinline__ float amp_sum(const Points &pts, const float *amp) {
 template <int SCATTER, class T, bool NT = true>
 struct FwdOp {
   static constexpr bool H = kScaled<T>;
-  static constexpr bool ONE_CELL = kSplit<T>;  // one cell per lane (VX = 1)
   static constexpr int NF = 2, NPRE = 4, NOUT = 3, NMAX = H ? 2 : 0;
   // FP16 levels hold s and r as one __half2 per cell (sr1: level k, sr0:
   // level k-1, srn: level k+1, the same buffer as sr0 in place)
@@ -231,7 +230,6 @@ struct FwdOp {
 // ------------------------------------------------------------- sourceProp
 struct SrcOp {
   static constexpr int NF = 1, NPRE = 2, NOUT = 1, NMAX = 0;
-  static constexpr bool ONE_CELL = false;
   static constexpr bool PAIRED = false;
   static constexpr bool Q2 = false;
   static constexpr bool DAMP = false;
@@ -271,7 +269,6 @@ struct SrcOp {
 // ------------------------------------------------------ reference adjoint
 struct RadjOp {
   static constexpr int NF = 2, NPRE = 5, NOUT = 3, NMAX = 0;
-  static constexpr bool ONE_CELL = false;
   static constexpr bool PAIRED = false;
   static constexpr bool Q2 = false;
   static constexpr bool DAMP = true;
@@ -345,7 +342,6 @@ struct RadjOp {
 template <int SCATTER, class T>
 struct SrecOp {
   static constexpr bool H = kScaled<T>;
-  static constexpr bool ONE_CELL = kSplit<T>;  // one cell per lane (VX = 1)
   static constexpr int NF = 1, NPRE = 2, NOUT = 2, NMAX = H ? 1 : 0;
   static constexpr bool PAIRED = false;
   static constexpr bool Q2 = H;  // steps in pairs (march_tile)
@@ -427,7 +423,6 @@ struct SrecOp {
 template <class T>
 struct LamOp {
   static constexpr bool H = kScaled<T>;
-  static constexpr bool ONE_CELL = kSplit<T>;  // one cell per lane (VX = 1)
   static constexpr int NF = 1, NPRE = 4, NOUT = 2, NMAX = H ? 1 : 0;
   static constexpr bool PAIRED = false;
   static constexpr bool Q2 = H;  // steps in pairs (march_tile)
@@ -686,16 +681,12 @@ inline Grid make_grid(const Geometry &geo, const TileShape &ts, int zc) {
 
 template <class Op, int RY, int WY, int VX, int LA, int LX>
 hipError_t launch_shape(const Geometry &geo, int zc, const Op &op, hipStream_t st) {
-  if constexpr (VX == 2 && Op::ONE_CELL) {
-    return hipErrorInvalidValue;
-  } else {
-    if (VX == 2 && geo.nx % 2) return hipErrorInvalidValue;
-    const Grid g = make_grid(geo, {RY, WY, VX, LA, LX}, zc);
-    const int nb = g.ntx * g.nty * g.ntz;
-    hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX, LA, LX>), dim3(nb), dim3(kTX * WY), 0, st,
-                       g, geo.taps, geo.sponge, op);
-    return hipGetLastError();
-  }
+  if (VX == 2 && geo.nx % 2) return hipErrorInvalidValue;
+  const Grid g = make_grid(geo, {RY, WY, VX, LA, LX}, zc);
+  const int nb = g.ntx * g.nty * g.ntz;
+  hipLaunchKernelGGL((march_kernel<Op, RY, WY, VX, LA, LX>), dim3(nb), dim3(kTX * WY), 0, st,
+                     g, geo.taps, geo.sponge, op);
+  return hipGetLastError();
 }
 
 template <class Op>
@@ -717,7 +708,6 @@ T *as(const void *p) {
 template <class BOp, int RY, int WY, int VX, int LA, int LX>
 hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchCommon &b,
                                      const BOp &op, hipStream_t st) {
-  if constexpr (VX == 2 && BOp::Op::ONE_CELL) return hipErrorInvalidValue;
   if (b.nshots < 1 || b.nshots > kMaxBatch) return hipErrorInvalidValue;
   BatchTiles bt{};
   bt.n = b.nshots;
@@ -731,9 +721,8 @@ hipError_t launch_batch_shape(const Taps &taps, int zc, const BatchCommon &b,
     tiles += g.ntx * g.nty * g.ntz;
   }
   bt.tile0[b.nshots] = tiles;
-  if constexpr (!(VX == 2 && BOp::Op::ONE_CELL))
-    hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX, LA, LX>), dim3(tiles),
-                       dim3(kTX * WY), 0, st, bt, taps, op);
+  hipLaunchKernelGGL((march_batch_kernel<BOp, RY, WY, VX, LA, LX>), dim3(tiles),
+                     dim3(kTX * WY), 0, st, bt, taps, op);
   return hipGetLastError();
 }
 

@@ -7,8 +7,9 @@ change one bit of the Born data or of the exact-adjoint image.  Each case
 pins the forward, or the background recompute and the adjoint-state kernel,
 to one shape on a domain that is not a multiple of any tile, with z chunks of
 8 and 13 planes (chunk edges next to the source and receiver planes), and
-compares with one fixed shape.  That shape's results are pinned to the CPU
-oracle by test_gpu_parity.py and test_gpu_fp16.py."""
+compares with one fixed shape, for FP32, FP16 and 24-bit levels.  That
+shape's results are pinned to the CPU oracle by test_gpu_parity.py,
+test_gpu_fp16.py and test_gpu_f24.py."""
 import numpy as np
 import pytest
 
@@ -55,25 +56,25 @@ def _check(got, want, what):
     assert np.array_equal(got, want), what
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "f24"])
 @pytest.mark.parametrize("lookahead", [1, 2])
 @pytest.mark.parametrize("tile", TILES)
 def test_forward_every_shape(pyborn, tile, lookahead, storage):
     c = _case()
-    kind = ["fwd" if storage == "f32" else "fwd_f16"]
+    kind = ["fwd" if storage == "f32" else "fwd_" + storage]
     want = _run(pyborn, c, storage, kind, REF, 64, None)
     for zc in (8, 13):
         got = _run(pyborn, c, storage, kind, tile + (lookahead,), zc, None)
         _check(got, want, "forward %s %s lookahead %d zc %d" % (storage, tile, lookahead, zc))
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "f24"])
 @pytest.mark.parametrize("lookahead", [1, 2])
 @pytest.mark.parametrize("tile", TILES)
 def test_adjoint_every_shape(pyborn, tile, lookahead, storage):
     c = _case()
     w = np.random.default_rng(11).standard_normal((c.nrec, c.ntb)).astype(np.float32)
-    sfx = "" if storage == "f32" else "_f16"
+    sfx = "" if storage == "f32" else "_" + storage
     kinds = ["background" + sfx, "adjoint_state" + sfx]
     want = _run(pyborn, c, storage, kinds, REF, 64, w)
     for zc in (8, 13):
