@@ -268,27 +268,30 @@ def pair_leg(args, pyborn, shots, tab, nt, torch):
                                  np.zeros((s.nrec, s.nt_big), np.float32))
         P.transfer_source_func(s.src_locs, s.src_vals)
 
-    # FP32 single-shot pairs; the launch shapes tuned on the first shot are
-    # fixed for the others (every shot has the same nz), so no tuning is timed
-    tuned, ref, t32 = None, [], 0.0
-    for k, s in enumerate(sel):
-        P = pyborn.Propagator(0)
+    # FP32 single-shot pairs on one context (its history buffer is allocated
+    # once, by an untimed pair on the largest shot); the launch shapes tuned
+    # on that pair are fixed for every shot (all have the same nz), so no
+    # tuning or allocation of the history is timed
+    P = pyborn.Propagator(0)
+    big = max(sel, key=lambda s: s.nx * s.ny * s.nz)
+    load(P, big)
+    m = torch.from_numpy(big.model).to(dev)
+    tr = torch.zeros((big.nt_big, big.nrec), dtype=torch.float32, device=dev)
+    img = torch.zeros(big.nx * big.ny * big.nz, dtype=torch.float32, device=dev)
+    P.dev_forward(1, nt, m, tr, scatter=0, keep_background=True)
+    P.dev_adjoint_exact(1, nt, tr, img, scatter=0)
+    P.synchronize()
+    tuned = {k2: v for k2, v in P.tuning().items()
+             if k2 in ("fwd", "background", "adjoint_state") and v["ms"] >= 0}
+    ref, t32 = [], 0.0
+    for s in sel:
         load(P, s)
+        for name, t in tuned.items():
+            P.set_tuning(name, t["rows_per_thread"], t["waves"], t["cells_per_lane"],
+                         t["lookahead"], t["zchunk"])
         m = torch.from_numpy(s.model).to(dev)
         tr = torch.zeros((s.nt_big, s.nrec), dtype=torch.float32, device=dev)
         img = torch.zeros(s.nx * s.ny * s.nz, dtype=torch.float32, device=dev)
-        if tuned is None:
-            P.dev_forward(1, nt, m, tr, scatter=0, keep_background=True)
-            P.dev_adjoint_exact(1, nt, tr, img, scatter=0)
-            P.synchronize()
-            tuned = {k2: v for k2, v in P.tuning().items()
-                     if k2 in ("fwd", "background", "adjoint_state") and v["ms"] >= 0}
-            tr.zero_()
-            img.zero_()
-        else:
-            for name, t in tuned.items():
-                P.set_tuning(name, t["rows_per_thread"], t["waves"], t["cells_per_lane"],
-                             t["lookahead"], t["zchunk"])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         P.dev_forward(1, nt, m, tr, scatter=0, keep_background=True)
@@ -296,9 +299,9 @@ def pair_leg(args, pyborn, shots, tab, nt, torch):
         P.synchronize()
         t32 += time.perf_counter() - t0
         ref.append(img.cpu().numpy())
-        P.close()
-        del m, tr, img
-        torch.cuda.empty_cache()
+    P.close()
+    del m, tr, img
+    torch.cuda.empty_cache()
     log("pair leg: FP32 one shot at a time %.2f s" % t32)
     out = {"workload": "Born forward + exact adjoint of the first %d of the windowed shots, "
                        "%d steps" % (len(sel), nt),

@@ -92,6 +92,12 @@ struct born_ctx {
   size_t budget = 0;
   float *d_seg = nullptr;
   size_t seg_cap = 0;  // fields
+  // The step kernels write interior cells only, so every level they read
+  // must hold zeros in its 4-cell shell.  seg_clean bytes from the start of
+  // d_seg are zero outside the interiors of levels of seg_nx x seg_ny x
+  // seg_nz cells of seg_esz bytes (cleared when that layout changes).
+  size_t seg_clean = 0, seg_esz = 0;
+  int seg_nx = -1, seg_ny = -1, seg_nz = -1;
   float *d_ckpt = nullptr;
   size_t ckpt_cap = 0;  // fields
   long long bg_version = 1;
@@ -221,10 +227,27 @@ void reset_tuning(born_ctx *c) {
   }
 }
 
+// A shape of two cells per lane needs an even nx.  For an odd nx (a shape
+// fixed by born_set_tuning or tuned on another grid) take the one-cell shape
+// of the same tile and lookahead, else 2 x 16 rows; every shape computes
+// the same bits, so only the speed changes.
+born::Tune fit_nx(born::Tune t, bool even_nx) {
+  const born::TileShape &ts = born::kShapes[t.shape];
+  if (ts.vx == 1 || even_nx) return t;
+  const int rows = ts.ry * (born::kTX * ts.wy / ts.lx);
+  int pick = 3;
+  for (int sh = born::kNumShapes - 1; sh >= 0; sh--) {
+    const born::TileShape &c = born::kShapes[sh];
+    if (c.vx == 1 && c.la == ts.la && c.ry * c.wy == rows) pick = sh;
+  }
+  t.shape = pick;
+  return t;
+}
+
 born::Tune tuning(const born_ctx *c, int kind) {
   born::Tune t = c->tune[kind];
   if (c->zchunk > 0) t.zchunk = c->zchunk;
-  return t;
+  return fit_nx(t, c->nx % 2 == 0);
 }
 
 // Autotuner: time every tile shape x z chunk for one step kernel and keep
@@ -496,13 +519,30 @@ size_t history_budget(born_ctx *ctx) {
 // history buffers: L levels of scatter operand + nck checkpoint levels, each
 // n elements of esz bytes (capacities are counted in floats)
 int history_alloc(born_ctx *ctx, size_t L, size_t nck, size_t esz) {
-  const size_t seg = (L * (size_t)ctx->n * esz + 3) / 4;
+  const size_t bytes = L * (size_t)ctx->n * esz;
+  const size_t seg = (bytes + 3) / 4;
   const size_t ck = (nck * (size_t)ctx->n * esz + 3) / 4;
   if (ctx->seg_cap < seg) {
     ctx->seg_cap = 0;
+    ctx->seg_clean = 0;
     ctx->hist_version = -1;
     CHK(dev_alloc(ctx, &ctx->d_seg, seg));
     ctx->seg_cap = seg;
+  }
+  // shells of the levels about to be used: zero unless cleared for this
+  // layout (levels of another grid or element size leave values there)
+  if (ctx->seg_nx != ctx->nx || ctx->seg_ny != ctx->ny || ctx->seg_nz != ctx->nz ||
+      ctx->seg_esz != esz) {
+    ctx->seg_clean = 0;
+    ctx->seg_nx = ctx->nx;
+    ctx->seg_ny = ctx->ny;
+    ctx->seg_nz = ctx->nz;
+    ctx->seg_esz = esz;
+  }
+  if (ctx->seg_clean < bytes) {
+    const size_t from = ctx->seg_clean / 4;
+    CHK(zero(ctx, ctx->d_seg + from, (long long)(seg - from)));
+    ctx->seg_clean = seg * 4;
   }
   if (ctx->ckpt_cap < ck) {
     ctx->ckpt_cap = 0;
@@ -758,7 +798,8 @@ int born_destroy(born_ctx *ctx) {
                   ctx->d_w[1],  ctx->d_w[2],  ctx->d_w[3], ctx->d_w2[0],
                   ctx->d_w2[1], ctx->d_w2[2], ctx->d_w2[3], ctx->d_img,
                   ctx->d_acc,   ctx->d_p0,    ctx->d_p1,   ctx->d_tr,
-                  ctx->d_tr2,   ctx->d_amp,   ctx->d_seg,  ctx->d_ckpt};
+                  ctx->d_tr2,   ctx->d_amp,   ctx->d_seg,  ctx->d_ckpt,
+                  ctx->d_ring,  ctx->d_hscale, ctx->d_ckring};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   (void)hipStreamDestroy(ctx->stream);
@@ -1789,6 +1830,9 @@ struct born_batch {
   float *d_acc = nullptr, *d_img = nullptr;
   float *d_seg = nullptr, *d_ckpt = nullptr;
   size_t seg_bytes = 0, ckpt_bytes = 0;
+  // bytes of d_seg whose level shells are zero for levels of seg_esz bytes
+  // per cell (the step kernels never write a shell)
+  size_t seg_clean = 0, seg_esz = 0;
   float *d_hscale = nullptr, *d_ckring = nullptr;
   size_t hscale_cap = 0, ckring_cap = 0;
   int last_segments = 0, last_recomputed = 0;
@@ -2063,7 +2107,7 @@ int born_batch_forward(born_batch *b, int jt, int nt, int scatter) {
   }
   born::Tune tu = b->tune[kind];
   if (ctx->zchunk > 0) tu.zchunk = ctx->zchunk;
-  if (born::kShapes[tu.shape].vx == 2 && !b->even_nx) tu.shape = 3;
+  tu = fit_nx(tu, b->even_nx);
   // initial state: fields and traces zero, amplitudes of every shot, scales
   for (int i = 0; i < 4; i++) CHK(zero(ctx, b->d_f[i], (long long)b->f_cap));
   CHK(zero(ctx, b->d_tr, b->tr_total));
@@ -2197,7 +2241,7 @@ int born_batch_adjoint_exact(born_batch *b, int jt, int nt, int scatter) {
   auto tune_of = [&](int kind) {
     born::Tune t = b->tune[kind];
     if (ctx->zchunk > 0) t.zchunk = ctx->zchunk;
-    if (born::kShapes[t.shape].vx == 2 && !b->even_nx) t.shape = 3;
+    t = fit_nx(t, b->even_nx);
     return t;
   };
   float *s_old = b->d_f[2], *s_cur = b->d_f[3];
@@ -2251,8 +2295,18 @@ int born_batch_adjoint_exact(born_batch *b, int jt, int nt, int scatter) {
   S = (nt + L - 1) / L;
   if (b->seg_bytes < (size_t)L * F) {
     b->seg_bytes = 0;
+    b->seg_clean = 0;
     CHK(dev_alloc(ctx, &b->d_seg, ((size_t)L * F + 3) / 4));
     b->seg_bytes = (size_t)L * F;
+  }
+  if (b->seg_esz != esz) {
+    b->seg_clean = 0;
+    b->seg_esz = esz;
+  }
+  if (b->seg_clean < (size_t)L * F) {
+    const size_t from = b->seg_clean / 4, to = ((size_t)L * F + 3) / 4;
+    CHK(zero(ctx, b->d_seg + from, (long long)(to - from)));
+    b->seg_clean = to * 4;
   }
   if (b->ckpt_bytes < 2 * (size_t)(S - 1) * F) {
     b->ckpt_bytes = 0;

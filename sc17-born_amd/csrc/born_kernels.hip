@@ -107,8 +107,6 @@ __global__ void transpose_kernel(float *dst, const float *src, int n0, int n1) {
 
 
 // the twelve tiles with one step of load lookahead, then with two
-
-// the twelve tiles with one step of load lookahead, then with two
 #define BORN_TILES(LA)                                                                     \
   {2, 4, 1, LA, 64}, {4, 4, 1, LA, 64}, {2, 8, 1, LA, 64}, {2, 16, 1, LA, 64},             \
       {1, 8, 1, LA, 64}, {1, 16, 1, LA, 64}, {1, 4, 2, LA, 32}, {2, 4, 2, LA, 32},         \

@@ -117,3 +117,64 @@ def test_exact_adjoint_refuses_zero_sponge_factor(pyborn):
     with pytest.raises(pyborn.BornError, match="UNSUPPORTED"):
         H.adjoint_exact(c.jt, rec, img, c.nt, 0)
     H.close()
+
+
+def _fresh_pair(pyborn, c, storage):
+    F = hip_prop(pyborn, c)
+    F.set_storage(storage)
+    out = _pair_run(F, c)
+    F.close()
+    return out
+
+
+def test_history_of_another_grid_leaves_no_values_in_level_shells(pyborn):
+    # The step kernels write level interiors only.  The history buffer kept
+    # from a larger grid holds interior values where the smaller grid's
+    # level shells lie; they must be cleared before the levels are read.
+    large = make_case(96, 72, 64, nt=40, nbound=8, seed=3)
+    small = make_case(60, 50, 44, nt=40, nbound=8, seed=4)
+    H = pyborn.Propagator(0)
+    _load(H, large)
+    _pair_run(H, large)
+    _load(H, small)
+    got = _pair_run(H, small)
+    H.close()
+    want = _fresh_pair(pyborn, small, "f32")
+    for g, w in zip(got, want):
+        assert np.abs(w).max() > 0
+        assert np.array_equal(g, w)
+
+
+def test_history_of_another_storage_leaves_no_values_in_level_shells(pyborn):
+    # 16-bit levels followed by FP32 levels in the same history buffer
+    c = make_case(64, 48, 40, nt=40, nbound=8, seed=5)
+    H = hip_prop(pyborn, c)
+    for storage in ("f16", "f24", "f32"):
+        H.set_storage(storage)
+        got = _pair_run(H, c)
+        want = _fresh_pair(pyborn, c, storage)
+        for g, w in zip(got, want):
+            assert np.abs(w).max() > 0
+            assert np.array_equal(g, w), storage
+    H.close()
+
+
+def test_batch_history_of_another_storage_leaves_no_values_in_level_shells(pyborn):
+    cases = [make_case(64, 48, 40, nt=50, nbound=10, seed=i, src_z=8 + i) for i in range(2)]
+    nt = cases[0].nt
+
+    def run(storages):
+        H = hip_prop(pyborn, cases[0])
+        B = pyborn.Batch(H, [_shot(c) for c in cases], cases[0].d)
+        for storage in storages:
+            H.set_storage(storage)
+            B.forward(1, nt, 0)
+            B.adjoint_exact(1, nt, 0)
+        out = [B.image(k) for k in range(len(cases))]
+        B.close()
+        H.close()
+        return out
+
+    for g, w in zip(run(["f16", "f32"]), run(["f32"])):
+        assert np.abs(w).max() > 0
+        assert np.array_equal(g, w)

@@ -80,3 +80,28 @@ def test_adjoint_every_shape(pyborn, tile, lookahead, storage):
     for zc in (8, 13):
         got = _run(pyborn, c, storage, kinds, tile + (lookahead,), zc, w)
         _check(got, want, "adjoint %s %s lookahead %d zc %d" % (storage, tile, lookahead, zc))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "f24"])
+def test_two_cell_shape_on_odd_nx_runs_one_cell_tile(pyborn, storage):
+    # a two-cell shape fixed on a grid of odd nx runs the one-cell shape of
+    # the same tile (two cells per lane need an even nx): same bits, no error
+    c = make_case(121, 43, 37, nt=30, nbound=8, src_z=9, rec_z=7, seed=4, jt=3)
+    w = np.random.default_rng(12).standard_normal((c.nrec, c.ntb)).astype(np.float32)
+    sfx = "" if storage == "f32" else "_" + storage
+    kinds = ["fwd" + sfx, "background" + sfx, "adjoint_state" + sfx]
+    outs = []
+    for shape in ((1, 4, 2, 1), REF):
+        H = hip_prop(pyborn, c)
+        H.set_storage(storage)
+        for k in kinds:
+            H.set_tuning(k, *shape, 13)
+        d = np.zeros((c.nrec, c.ntb), np.float32)
+        H.transfer_receiver_func(c.rec_nx, c.rec_ny, c.rec_locs, d)
+        H.forward_scatter(c.jt, c.model, d, c.nt, 0)
+        g = np.zeros(c.nx * c.ny * c.nz, np.float32)
+        H.adjoint_exact(c.jt, w, g, c.nt, scatter=0)
+        H.close()
+        outs.append((d, g))
+    for got, want, what in zip(outs[0], outs[1], ("data", "image")):
+        _check(got, want, "%s %s odd nx" % (what, storage))
