@@ -160,9 +160,17 @@ int born_dev_source_prop_noalloc(born_ctx *ctx, int jts, int nt, float *p0, floa
  * 0 = all free device memory less a reserve.  The number of checkpoints is
  * the smallest that fits (1 segment = the whole history kept). */
 int born_set_checkpoint_budget(born_ctx *ctx, size_t bytes);
-/* segments and recomputed steps of the last exact adjoint */
+/* segments and recomputed steps of the last exact adjoint.  A Born forward
+ * asked to keep the background (FP32 storage) whose history does not fit
+ * keeps the checkpoint plan instead: checkpoints at segment starts and the
+ * last segment, so the next exact adjoint of the same background, scatter,
+ * jt and nt recomputes only the earlier segments. */
 int born_last_checkpoint_plan(const born_ctx *ctx, int *segments,
                               int *recomputed_steps);
+/* background steps the last exact adjoint swept before its segments (nt when
+ * it found neither the history nor the checkpoint plan of a forward, else
+ * 0); no counterpart in the reference, which keeps no checkpoints */
+int born_last_background_sweep(const born_ctx *ctx);
 /* Storage of the wavefield levels and background history of the Born
  * forward and exact adjoint (born_dev_forward, born_dev_adjoint_exact and
  * their host-pointer forms).  BORN_STORAGE_F16 keeps every level as FP16

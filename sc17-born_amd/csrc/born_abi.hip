@@ -103,7 +103,12 @@ struct born_ctx {
   long long bg_version = 1;
   long long hist_version = -1;
   int hist_scatter = -1, hist_jt = -1, hist_nt = -1, hist_storage = -1;
-  int last_segments = 0, last_recomputed = 0;
+  // a checkpoint plan of S segments of L levels left by the forward (FP32
+  // levels): checkpoints of segments 1..S-1 in d_ckpt, the last segment's
+  // scatter operands in d_seg; valid for ck_version == bg_version
+  long long ck_version = -1;
+  int ck_S = 0, ck_L = 0, ck_scatter = -1, ck_jt = -1, ck_nt = -1;
+  int last_segments = 0, last_recomputed = 0, last_swept = 0;
   int last_pairs = 0;
   int last_lam_pairs = 0;
 
@@ -549,6 +554,27 @@ int history_alloc(born_ctx *ctx, size_t L, size_t nck, size_t esz) {
     CHK(dev_alloc(ctx, &ctx->d_ckpt, ck));
     ctx->ckpt_cap = ck;
   }
+  return BORN_OK;
+}
+
+// The checkpoint plan of an exact adjoint over nt steps: the fewest
+// segments S whose L = ceil(nt / S) levels of scatter operand plus 2 (S - 1)
+// checkpoint levels fit the history budget.
+int checkpoint_plan(born_ctx *ctx, int nt, size_t esz, int *S, int *L) {
+  const size_t F = (size_t)ctx->n * esz;
+  const size_t avail = history_budget(ctx);
+  int segs = 0;
+  for (int sg = 1; sg <= nt; sg++) {
+    const size_t l = (size_t)((nt + sg - 1) / sg);
+    if ((l + 2 * (size_t)(sg - 1)) * F <= avail) {
+      segs = sg;
+      break;
+    }
+    if (l <= 2) break;
+  }
+  if (segs == 0) return fail(ctx, BORN_ERR_NOMEM, "no checkpoint plan fits the HBM budget");
+  *L = (nt + segs - 1) / segs;
+  *S = (nt + *L - 1) / *L;
   return BORN_OK;
 }
 
@@ -1030,14 +1056,23 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   const bool try_pairs = ctx->pairs && !half && nt >= 2;
   if (try_pairs) CHK(ensure_pair_work(ctx));
   float *hist = nullptr;
+  // With FP32 levels a history that does not fit becomes the adjoint's
+  // checkpoint plan: checkpoints at segment starts and the last segment's
+  // scatter operands, so that the adjoint need not sweep the background
+  // first.  (Scaled levels keep s and r together; their adjoint sweeps.)
+  int ckS = 0, ckL = 0;
   if (keep_background && nt > 0) {
     if ((size_t)nt * n * esz <= history_budget(ctx)) {
       CHK(history_alloc(ctx, (size_t)nt, 0, esz));
       hist = ctx->d_seg;
       if (half) CHK(ensure(ctx, &ctx->d_hscale, &ctx->hscale_cap, (size_t)nt));
+    } else if (!half && !try_pairs) {
+      CHK(checkpoint_plan(ctx, nt, esz, &ckS, &ckL));
+      CHK(history_alloc(ctx, (size_t)ckL, 2 * (size_t)(ckS - 1), esz));
     }
   }
   ctx->hist_version = -1;
+  ctx->ck_version = -1;
   ctx->jt_state = jt;
   const Geometry geo = geometry(ctx);
   const int kind = fwd_kind(ctx->storage, scatter);
@@ -1145,13 +1180,39 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   // keep s and r together as one __half2 per cell in sb[], and the history
   // is written apart.)
   const bool s_in_hist = hist && scatter == 0 && !half;
+  // checkpoint plan: levels it and it-1 saved before step it at segment
+  // starts it = c L (c = 1..S-1); from klast on the scatter operands go to
+  // the segment history (scatter 0: the s levels themselves, level j > klast
+  // in segment level j-1-klast, levels klast and klast-1 staying in s_cur
+  // and s_old)
+  const int klast = ckS > 1 ? (ckS - 1) * ckL : nt;
+  const size_t Fb = (size_t)n * esz;
   for (; it < nt; it++) {
     const int id = it / jt, ii = it - id * jt;
+    if (ckS > 1 && it > 0 && it % ckL == 0 && it / ckL <= ckS - 1) {
+      const long long c = it / ckL - 1;
+      HIPCHK(hipMemcpyAsync(nth(ctx->d_ckpt, 2 * c, n, esz), s_cur, Fb,
+                            hipMemcpyDeviceToDevice, ctx->stream));
+      HIPCHK(hipMemcpyAsync(nth(ctx->d_ckpt, 2 * c + 1, n, esz), s_old, Fb,
+                            hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    const bool into_seg = it >= klast;
     if (s_in_hist) {
       a.s1 = it >= 1 ? nth(hist, it - 1, n, esz) : s_cur;  // level 0: zero
       a.s0 = it >= 2 ? nth(hist, it - 2, n, esz) : s_old;
       a.s_new = nth(hist, it, n, esz);
       a.hist = nullptr;
+    } else if (into_seg && scatter == 0) {
+      a.s1 = it > klast ? nth(ctx->d_seg, it - 1 - klast, n, esz) : s_cur;
+      a.s0 = it > klast + 1 ? nth(ctx->d_seg, it - 2 - klast, n, esz)
+                            : (it > klast ? s_cur : s_old);
+      a.s_new = nth(ctx->d_seg, it - klast, n, esz);
+      a.hist = nullptr;
+    } else if (into_seg) {
+      a.s1 = s_cur;
+      a.s0 = s_old;
+      a.s_new = nullptr;
+      a.hist = nth(ctx->d_seg, it - klast, n, esz);
     } else {
       a.s1 = s_cur;
       a.s0 = s_old;
@@ -1167,7 +1228,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     a.extract = id + 7 < ctx->nt_rec;
     a.h = half_state(ctx, it);
     HIPCHK(born::launch_fwd_step(geo, tu, a, ctx->stream));
-    std::swap(s_old, s_cur);
+    if (!(into_seg && scatter == 0)) std::swap(s_old, s_cur);
     std::swap(r_old, r_cur);
   }
   if (hist) {
@@ -1176,6 +1237,14 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     ctx->hist_jt = jt;
     ctx->hist_nt = nt;
     ctx->hist_storage = ctx->storage;
+  }
+  if (ckS > 1) {
+    ctx->ck_version = ctx->bg_version;
+    ctx->ck_S = ckS;
+    ctx->ck_L = ckL;
+    ctx->ck_scatter = scatter;
+    ctx->ck_jt = jt;
+    ctx->ck_nt = nt;
   }
   return BORN_OK;
 }
@@ -1380,24 +1449,19 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
                       ctx->hist_scatter == scatter && ctx->hist_jt == jt &&
                       ctx->hist_nt == nt && ctx->hist_storage == ctx->storage &&
                       ctx->seg_cap * sizeof(float) >= (size_t)nt * n * esz;
+  // the checkpoints and last segment that the forward left (FP32 levels)
+  const bool planned = !cached && !half && ctx->ck_version == ctx->bg_version &&
+                       ctx->ck_scatter == scatter && ctx->ck_jt == jt && ctx->ck_nt == nt;
   int S = 1, L = nt;
-  if (!cached) {
+  if (planned) {
+    S = ctx->ck_S;
+    L = ctx->ck_L;
+  }
+  // this adjoint overwrites the segment history
+  ctx->ck_version = -1;
+  if (!cached && !planned) {
     ctx->hist_version = -1;  // the segment buffer is about to be reused
-    const size_t F = (size_t)n * esz;
-    const size_t avail = history_budget(ctx);
-    S = 0;
-    for (int s = 1; s <= nt; s++) {
-      const size_t l = (size_t)((nt + s - 1) / s);
-      if ((l + 2 * (size_t)(s - 1)) * F <= avail) {
-        S = s;
-        break;
-      }
-      if (l <= 2) break;
-    }
-    if (S == 0)
-      return fail(ctx, BORN_ERR_NOMEM, "no checkpoint plan fits the HBM budget");
-    L = (nt + S - 1) / S;
-    S = (nt + L - 1) / L;
+    CHK(checkpoint_plan(ctx, nt, esz, &S, &L));
     CHK(history_alloc(ctx, (size_t)L, 2 * (size_t)(S - 1), esz));
     if (half) {
       CHK(ensure(ctx, &ctx->d_hscale, &ctx->hscale_cap, (size_t)L));
@@ -1416,7 +1480,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   sr.ring = ring(ctx, RING_S);
   const size_t Fb = (size_t)n * esz;
   int recomputed = 0;
-  if (!cached) {
+  if (!cached && !planned) {
     // pass 1: background sweep; checkpoints at segment starts (with the
     // FP16 scale ring), the last segment's scatter operands stored directly
     CHK(zero(ctx, s_old, n));
@@ -1560,6 +1624,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
                                  scatter == 0 ? 1e-6f : 1.f, ctx->stream));
   ctx->last_segments = S;
   ctx->last_recomputed = recomputed;
+  ctx->last_swept = !cached && !planned ? nt : 0;
   return BORN_OK;
 }
 
@@ -1677,6 +1742,11 @@ int born_last_checkpoint_plan(const born_ctx *ctx, int *segments, int *recompute
   if (segments) *segments = ctx->last_segments;
   if (recomputed) *recomputed = ctx->last_recomputed;
   return BORN_OK;
+}
+
+int born_last_background_sweep(const born_ctx *ctx) {
+  if (!ctx) return BORN_ERR_ARG;
+  return ctx->last_swept;
 }
 
 int born_set_zchunk(born_ctx *ctx, int zchunk) {

@@ -64,8 +64,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1005)
     args = ap.parse_args()
 
-    import torch  # noqa: F401  (one HIP runtime with torch, see pyborn)
-
+    # host buffers only: no torch in this process, so the library brings
+    # the only HIP runtime (and a fresh machine skips torch's start-up)
     pyborn = _load("pyborn")
     configs = _load("configs")
     mx, my, mz = args.n

@@ -55,6 +55,7 @@ SIGNATURES = {
     "born_dev_source_prop_noalloc": (_i, [_vp, _i, _i, _vp, _vp]),
     "born_set_checkpoint_budget": (_i, [_vp, C.c_size_t]),
     "born_last_checkpoint_plan": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
+    "born_last_background_sweep": (_i, [_vp]),
     "born_set_zchunk": (_i, [_vp, _i]),
     "born_set_storage": (_i, [_vp, _i]),
     "born_set_step_pairs": (_i, [_vp, _i]),
@@ -229,6 +230,13 @@ class Propagator:
         s, r = C.c_int(), C.c_int()
         self._chk(self.lib.born_last_checkpoint_plan(self.h, C.byref(s), C.byref(r)), "plan")
         return s.value, r.value
+
+    def last_background_sweep(self):
+        """background steps the last exact adjoint swept before its
+        segments (0 when the forward left the history or the checkpoints)"""
+        n = self.lib.born_last_background_sweep(self.h)
+        self._chk(min(n, 0), "background sweep")
+        return n
 
     def set_zchunk(self, zc):
         self._chk(self.lib.born_set_zchunk(self.h, zc), "zchunk")

@@ -159,6 +159,63 @@ def test_exact_adjoint_checkpoint_plans_agree(pyborn):
 
 
 @pytest.mark.parametrize("scatter", [0, 1])
+def test_forward_leaves_checkpoint_plan_for_adjoint(pyborn, scatter):
+    # a forward asked to keep a background that does not fit the budget
+    # keeps the checkpoints and the last segment instead; the adjoint then
+    # sweeps nothing and gives bitwise the image of the full history
+    torch = pytest.importorskip("torch")
+    c = make_case(48, 40, 32, nt=70, nbound=8, seed=6)
+    dev = torch.device("cuda:0")
+    model = torch.from_numpy(c.model.ravel().copy()).to(dev)
+    w = torch.from_numpy(np.random.default_rng(4).standard_normal((c.ntb, c.nrec))
+                         .astype(np.float32)).to(dev)
+    H = hip_prop(pyborn, c)
+    H.transfer_receiver_func(c.rec_nx, c.rec_ny, c.rec_locs, np.zeros((c.nrec, c.ntb), np.float32))
+
+    def pair(keep, adjoints=1):
+        tr = torch.zeros((c.ntb, c.nrec), dtype=torch.float32, device=dev)
+        H.dev_forward(c.jt, c.nt, model, tr, scatter=scatter, keep_background=keep)
+        imgs, sweeps = [], []
+        for _ in range(adjoints):
+            img = torch.zeros(c.nx * c.ny * c.nz, dtype=torch.float32, device=dev)
+            H.dev_adjoint_exact(c.jt, c.nt, w, img, scatter=scatter)
+            H.synchronize()
+            imgs.append(img.cpu().numpy())
+            sweeps.append(H.last_background_sweep())
+        return tr.cpu().numpy(), imgs, sweeps
+
+    want_tr, (want,), _ = pair(True)
+    assert H.last_checkpoint_plan() == (1, 0)
+    assert np.abs(want).max() > 0
+    F = c.nx * c.ny * c.nz * 4
+    for budget_fields, expect_segments in [(40, 2), (25, 4)]:
+        H.set_checkpoint_budget(budget_fields * F)
+        tr, imgs, sweeps = pair(True, adjoints=2)
+        segs, recomputed = H.last_checkpoint_plan()
+        L = -(-c.nt // segs)
+        assert segs == expect_segments and recomputed == (segs - 1) * L
+        # the first adjoint uses the forward's plan; the second, whose
+        # segment history the first overwrote, sweeps the background again
+        assert sweeps == [0, c.nt]
+        assert np.array_equal(tr, want_tr)
+        for g in imgs:
+            assert np.array_equal(g, want)
+        # a forward that keeps nothing leaves no plan
+        _, (g,), sweeps = pair(False)
+        assert sweeps == [c.nt] and np.array_equal(g, want)
+    # a new source after the forward invalidates its plan
+    H.set_checkpoint_budget(25 * F)
+    tr = torch.zeros((c.ntb, c.nrec), dtype=torch.float32, device=dev)
+    H.dev_forward(c.jt, c.nt, model, tr, scatter=scatter, keep_background=True)
+    H.transfer_source_func(c.src_locs, 2 * c.src_vals)
+    img = torch.zeros(c.nx * c.ny * c.nz, dtype=torch.float32, device=dev)
+    H.dev_adjoint_exact(c.jt, c.nt, w, img, scatter=scatter)
+    H.synchronize()
+    assert H.last_background_sweep() == c.nt
+    H.close()
+
+
+@pytest.mark.parametrize("scatter", [0, 1])
 def test_dot_product(pyborn, scatter):
     """<F m, d> = <m, F^T d> with FP64 sums (my_operator.cc:13-62 asks for
     1e-4; the target here is 1e-5)."""
