@@ -870,9 +870,10 @@ int born_create_space(born_ctx *ctx, float d1, float d2, float d3, float bc_a,
     CHK(dev_alloc(ctx, &ctx->d_acc, 0));
     CHK(dev_alloc(ctx, &ctx->d_p0, 0));
     CHK(dev_alloc(ctx, &ctx->d_p1, 0));
-    CHK(dev_alloc(ctx, &ctx->d_seg, 0));
-    CHK(dev_alloc(ctx, &ctx->d_ckpt, 0));
-    ctx->seg_cap = ctx->ckpt_cap = 0;
+    // The history and checkpoint buffers stay: freeing a buffer of hundreds
+    // of GB and allocating it again took 5-7 s (the driver clears released
+    // memory), a whole shot's work.  history_alloc clears the level shells
+    // for the new layout and grows the buffers when the new grid needs more.
     CHK(dev_alloc(ctx, &ctx->d_vel1, 0));
     CHK(dev_alloc(ctx, &ctx->d_vel2, 0));
     CHK(dev_alloc(ctx, &ctx->d_rmap, 0));
