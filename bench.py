@@ -375,6 +375,10 @@ def main():
     ap.add_argument("--fwd-shape", default="",
                     help="fix the FP32 forward launch shape: rows,waves,cells_per_lane"
                          "[,lookahead] (64-cell rows; profiling runs), e.g. 1,8,1 or 2,8,1,2")
+    ap.add_argument("--tuning", default="",
+                    help="fix the launch shapes of the single-shot kernels: a JSON file or "
+                         "string in the form of config.tuning of an earlier line (profiling "
+                         "runs that must launch the shapes a full run chose)")
     ap.add_argument("--step-pairs", type=int, default=0,
                     help="FP32 forward in two-step launches: 0 off, 1 when timed faster, 2 on")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -420,6 +424,13 @@ def main():
     if args.zchunk:
         P.set_zchunk(args.zchunk)
     P.set_step_pairs(args.step_pairs)
+    if args.tuning:
+        text = open(args.tuning).read() if os.path.exists(args.tuning) else args.tuning
+        fixed = json.loads(text)
+        fixed = fixed.get("config", fixed).get("tuning", fixed)
+        for name, t in fixed.items():
+            P.set_tuning(name, t["rows_per_thread"], t["waves"], t["cells_per_lane"],
+                         t["lookahead"], t["zchunk"])
     if args.fwd_shape:
         sh = [int(v) for v in args.fwd_shape.split(",")]
         ry, wy, vx, la = sh + [1] * (4 - len(sh))

@@ -181,12 +181,12 @@ int born_last_background_sweep(const born_ctx *ctx);
  * FP32. */
 #define BORN_STORAGE_F32 0
 #define BORN_STORAGE_F16 1
-/* BORN_STORAGE_F24: every level as an FP16 value plus a refinement byte per
- * cell (3 bytes), with the same per-level scales as BORN_STORAGE_F16: every
- * value keeps 19 significant bits instead of FP16's 11, so the rounding
- * noise injected at every step no longer dominates the error
- * of thousands of steps.  Fields of 24-bit levels hold fewer than
- * 715,827,882 cells. */
+/* BORN_STORAGE_F24: every level as its FP32 value rounded to 24 bits (3
+ * bytes per cell: sign, the 8 exponent bits and 15 significand bits, the top
+ * 16 bits and the byte below them stored apart); relative precision 2^-16
+ * over the whole FP32 range, no scale, so the rounding noise injected at
+ * every step stays below 1e-3 of the result over thousands of steps.
+ * Fields of 24-bit levels hold fewer than 715,827,882 cells. */
 #define BORN_STORAGE_F24 2
 int born_set_storage(born_ctx *ctx, int storage);
 

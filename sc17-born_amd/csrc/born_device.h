@@ -351,23 +351,21 @@ __device__ __forceinline__ void st_pair(__half2 *p, uint32_t o, const float (&a)
   }
 }
 
-// ---- 24-bit storage: an FP16 value plus a refinement byte per cell.  The
-// level is scaled as in FP16 storage (|value * scale| < 16384 by the level's
-// bound).  w = value * scale is rounded to 19 significant bits (half-way
-// cases away from zero); hi holds its top 11 bits as an FP16 value and lo the
-// next 8 bits of the FP32 significand, so hi's FP32 bits with lo shifted in
-// below them are the rounded w.  The relative precision is 2^-19 at every
-// magnitude (FP16 alone: 2^-11), so the rounding noise injected at every step
-// no longer dominates the error of thousands of steps.  (Below FP16's normal
-// range, 2^-27 of the bound, hi is rounded toward zero and lo perturbs it by
-// less than 2^-10 of itself.)
+// ---- 24-bit storage: an FP32 value rounded to 24 bits per cell (sign, the
+// 8 exponent bits and 15 significand bits; relative precision 2^-16, the
+// whole FP32 range, no scale).  hi holds the top 16 bits of the rounded
+// FP32 word and lo the 8 bits below them; the value's FP32 bits are
+// hi << 16 | lo << 8.  Rounding adds 0x80 to the word and truncates: half
+// way cases go away from zero, a finite value past the largest 24-bit one
+// becomes an infinity as in FP32 rounding, infinities stay infinities.
 //
-// Layout of a level of n cells: the n hi values, then the n lo bytes; for two
-// fields stored together (s, r), n __half2 then n byte pairs.  So a cell's
-// hi and lo addresses are a shift of its offset, lo's plus the uniform
-// 2n (4n) bytes.  A pointer to Split (SplitPair) advances by 3 (6) bytes per
-// cell: the levels of a history and the shots of a batch are whole regions
-// of 3n (6n) bytes.
+// Layout of a level of n cells: the n hi halves, then the n lo bytes; for
+// two fields stored together (s, r), n hi pairs then n lo byte pairs.  So
+// a cell's hi and lo addresses are a shift of its offset, lo's plus the
+// uniform 2n (4n) bytes.  A pointer to Split (SplitPair) advances by 3 (6)
+// bytes per cell: the levels of a history and the shots of a batch are
+// whole regions of 3n (6n) bytes.  The scale arguments of the loads and
+// stores below are those of the scaled FP16 storage and are not used.
 struct Split {
   unsigned char b[3];
 };
@@ -376,37 +374,28 @@ struct SplitPair {
 };
 static_assert(sizeof(Split) == 3 && sizeof(SplitPair) == 6, "packed split types");
 
-// value of one stored cell: (hi with lo's 8 bits below it) / scale
-__device__ __forceinline__ float split_val(uint32_t hi_f32_bits, uint32_t lo, float inv) {
-  return mul_r(__uint_as_float(hi_f32_bits | (lo & 0xffu) << 5), inv);
-}
-// w = value * scale rounded to 19 bits: the 11-bit top part (FP16-exact)
-// and the byte below it
-__device__ __forceinline__ float split_top(float v, float scale, uint32_t &lo) {
-  const float w = fminf(fmaxf(mul_r(v, scale), -65504.f), 65504.f);
-  const uint32_t q = __float_as_uint(w) + 0x10u;  // round at the 19th bit
-  lo = (q >> 5) & 0xffu;
-  return __uint_as_float(q & 0xffffe000u);
-}
-__device__ __forceinline__ uint32_t f16_f32_bits(unsigned short h) {
-  return __float_as_uint(__half2float(__ushort_as_half(h)));
+// the rounded FP32 word of v
+__device__ __forceinline__ uint32_t split_word(float v) { return __float_as_uint(v) + 0x80u; }
+// value of a stored cell from hi (in the low 16 bits) and lo (low 8 bits)
+__device__ __forceinline__ float split_val(uint32_t hi, uint32_t lo) {
+  return __uint_as_float(hi << 16 | (lo & 0xffu) << 8);
 }
 
 // VX = 2: two consecutive cells per lane (o even, n even: a pair of hi
 // values is 4-byte aligned, its lo bytes 2-byte aligned)
 template <int VX>
-__device__ __forceinline__ void ld_vec(const Split *p, uint32_t o, float inv, float (&v)[VX],
+__device__ __forceinline__ void ld_vec(const Split *p, uint32_t o, float, float (&v)[VX],
                                        uint32_t n) {
   const char *b = (const char *)p;
   if constexpr (VX == 1) {
-    const unsigned short hi = *(const unsigned short *)(b + o * 2u);
+    const uint32_t hi = *(const unsigned short *)(b + o * 2u);
     const uint32_t lo = *(const unsigned char *)(b + (2u * n + o));
-    v[0] = split_val(f16_f32_bits(hi), lo, inv);
+    v[0] = split_val(hi, lo);
   } else {
     const uint32_t hi = *(const uint32_t *)(b + o * 2u);
     const uint32_t lo = *(const unsigned short *)(b + (2u * n + o));
-    v[0] = split_val(f16_f32_bits((unsigned short)(hi & 0xffffu)), lo, inv);
-    v[1] = split_val(f16_f32_bits((unsigned short)(hi >> 16)), lo >> 8, inv);
+    v[0] = __uint_as_float(hi << 16 | (lo & 0xffu) << 8);
+    v[1] = __uint_as_float((hi & 0xffff0000u) | (lo & 0xff00u));
   }
 }
 template <int VX>
@@ -415,65 +404,53 @@ __device__ __forceinline__ void ld_stream(const Split *p, uint32_t o, float inv,
   ld_vec<VX>(p, o, inv, v, n);
 }
 template <int VX, bool NT = true>
-__device__ __forceinline__ void st_vec(Split *p, uint32_t o, const float (&v)[VX], float scale,
+__device__ __forceinline__ void st_vec(Split *p, uint32_t o, const float (&v)[VX], float,
                                        uint32_t n) {
-  uint32_t l0, l1 = 0;
-  const float t0 = split_top(v[0], scale, l0);
-  // top is FP16-exact where it matters; rounded toward zero below FP16's range
-  float t1 = 0.f;
-  if constexpr (VX == 2) t1 = split_top(v[1], scale, l1);
-  const auto h = __builtin_amdgcn_cvt_pkrtz(t0, t1);
-  uint32_t hb;
-  __builtin_memcpy(&hb, &h, 4);
+  const uint32_t q0 = split_word(v[0]);
   if constexpr (VX == 1) {
-    st_b16<NT>(p, o * 2u, (unsigned short)(hb & 0xffffu));
-    st_b8<NT>(p, 2u * n + o, (unsigned char)l0);
+    st_b16<NT>(p, o * 2u, (unsigned short)(q0 >> 16));
+    st_b8<NT>(p, 2u * n + o, (unsigned char)(q0 >> 8));
   } else {
-    st_b32<NT>(p, o * 2u, hb);
-    st_b16<NT>(p, 2u * n + o, (unsigned short)(l0 | l1 << 8));
+    const uint32_t q1 = split_word(v[1]);
+    st_b32<NT>(p, o * 2u, q0 >> 16 | (q1 & 0xffff0000u));
+    st_b16<NT>(p, 2u * n + o, (unsigned short)((q0 >> 8 & 0xffu) | (q1 & 0xff00u)));
   }
 }
 // two fields stored together: first field in the low half of each pair
 template <int VX>
-__device__ __forceinline__ void ld_pair(const SplitPair *p, uint32_t o, const float (&inv)[2],
+__device__ __forceinline__ void ld_pair(const SplitPair *p, uint32_t o, const float (&)[2],
                                         float (&v)[2][VX], uint32_t n) {
   const char *b = (const char *)p;
   if constexpr (VX == 1) {
     const uint32_t hi = *(const uint32_t *)(b + o * 4u);
     const uint32_t lo = *(const unsigned short *)(b + (4u * n + o * 2u));
-    v[0][0] = split_val(f16_f32_bits((unsigned short)(hi & 0xffffu)), lo, inv[0]);
-    v[1][0] = split_val(f16_f32_bits((unsigned short)(hi >> 16)), lo >> 8, inv[1]);
+    v[0][0] = __uint_as_float(hi << 16 | (lo & 0xffu) << 8);
+    v[1][0] = __uint_as_float((hi & 0xffff0000u) | (lo & 0xff00u));
   } else {
     const uint2 hi = *(const uint2 *)(b + o * 4u);
     const uint32_t lo = *(const uint32_t *)(b + (4u * n + o * 2u));
-    v[0][0] = split_val(f16_f32_bits((unsigned short)(hi.x & 0xffffu)), lo, inv[0]);
-    v[1][0] = split_val(f16_f32_bits((unsigned short)(hi.x >> 16)), lo >> 8, inv[1]);
-    v[0][1] = split_val(f16_f32_bits((unsigned short)(hi.y & 0xffffu)), lo >> 16, inv[0]);
-    v[1][1] = split_val(f16_f32_bits((unsigned short)(hi.y >> 16)), lo >> 24, inv[1]);
+    v[0][0] = __uint_as_float(hi.x << 16 | (lo & 0xffu) << 8);
+    v[1][0] = __uint_as_float((hi.x & 0xffff0000u) | (lo & 0xff00u));
+    v[0][1] = __uint_as_float(hi.y << 16 | (lo >> 8 & 0xff00u));
+    v[1][1] = __uint_as_float((hi.y & 0xffff0000u) | (lo >> 16 & 0xff00u));
   }
 }
 template <int VX, bool NT = true>
 __device__ __forceinline__ void st_pair(SplitPair *p, uint32_t o, const float (&a)[VX],
-                                        const float (&b)[VX], const float (&scale)[2],
+                                        const float (&b)[VX], const float (&)[2],
                                         uint32_t n) {
-  uint32_t la0, lb0;
-  const float ta0 = split_top(a[0], scale[0], la0);
-  const float tb0 = split_top(b[0], scale[1], lb0);
-  const auto h0 = __builtin_amdgcn_cvt_pkrtz(ta0, tb0);
-  uint32_t hb0;
-  __builtin_memcpy(&hb0, &h0, 4);
+  const uint32_t qa0 = split_word(a[0]), qb0 = split_word(b[0]);
+  const uint32_t h0 = qa0 >> 16 | (qb0 & 0xffff0000u);
+  const uint32_t l0 = (qa0 >> 8 & 0xffu) | (qb0 & 0xff00u);
   if constexpr (VX == 1) {
-    st_b32<NT>(p, o * 4u, hb0);
-    st_b16<NT>(p, 4u * n + o * 2u, (unsigned short)(la0 | lb0 << 8));
+    st_b32<NT>(p, o * 4u, h0);
+    st_b16<NT>(p, 4u * n + o * 2u, (unsigned short)l0);
   } else {
-    uint32_t la1, lb1;
-    const float ta1 = split_top(a[1], scale[0], la1);
-    const float tb1 = split_top(b[1], scale[1], lb1);
-    const auto h1 = __builtin_amdgcn_cvt_pkrtz(ta1, tb1);
-    uint32_t hb1;
-    __builtin_memcpy(&hb1, &h1, 4);
-    st_b64<NT>(p, o * 4u, (uint64_t)hb1 << 32 | hb0);
-    st_b32<NT>(p, 4u * n + o * 2u, la0 | lb0 << 8 | la1 << 16 | lb1 << 24);
+    const uint32_t qa1 = split_word(a[1]), qb1 = split_word(b[1]);
+    const uint32_t h1 = qa1 >> 16 | (qb1 & 0xffff0000u);
+    const uint32_t l1 = (qa1 >> 8 & 0xffu) | (qb1 & 0xff00u);
+    st_b64<NT>(p, o * 4u, (uint64_t)h1 << 32 | h0);
+    st_b32<NT>(p, 4u * n + o * 2u, l0 | l1 << 16);
   }
 }
 

@@ -42,15 +42,19 @@ __device__ __forceinline__ int rec_first(const RecMap &rm, int slot, int x, int 
   return rm.rmap[((long long)slot * ny + y) * nx + x];
 }
 
-// Scaled storage (FP16, or FP16 with a refinement byte) shares the state of
-// the ops (unused with float storage): each stored level has a power-of-two
-// scale derived from a bound on |new level|, gain * max|cur| + max|old| +
-// injections, with gain = 2 + max(V) * tapsum and tapsum >= |L u| / max|u|.
+// Scaled storage (FP16) shares the state of the ops (unused with float and
+// 24-bit storage): each stored level has a power-of-two scale derived from a
+// bound on |new level|, gain * max|cur| + max|old| + injections, with
+// gain = 2 + max(V) * tapsum and tapsum >= |L u| / max|u|.
 template <class T>
-constexpr bool kScaled = std::is_same<T, __half>::value || std::is_same<T, Split>::value;
-// 24-bit storage (FP16 with a refinement byte)
+constexpr bool kScaled = std::is_same<T, __half>::value;
+// 24-bit storage (FP32 rounded to 24 bits)
 template <class T>
 constexpr bool kSplit = std::is_same<T, Split>::value;
+// levels narrower than FP32 (FP16 or 24-bit): the forward stores s and r
+// together, the steps run in pairs, the streams take the FP16 cache hints
+template <class T>
+constexpr bool kNarrow = !std::is_same<T, float>::value;
 // two fields stored together: FP16 as __half2, 24-bit as SplitPair
 template <class T>
 using PairOf = typename std::conditional<kSplit<T>, SplitPair, __half2>::type;
@@ -77,10 +81,10 @@ template <int SCATTER, class T, bool NT = true>
 struct FwdOp {
   static constexpr bool H = kScaled<T>;
   static constexpr int NF = 2, NPRE = 4, NOUT = 3, NMAX = H ? 2 : 0;
-  // FP16 levels hold s and r as one __half2 per cell (sr1: level k, sr0:
-  // level k-1, srn: level k+1, the same buffer as sr0 in place)
-  static constexpr bool PAIRED = H;
-  static constexpr bool Q2 = H;  // steps in pairs (march_tile)
+  // FP16 and 24-bit levels hold s and r together per cell (sr1: level k,
+  // sr0: level k-1, srn: level k+1, the same buffer as sr0 in place)
+  static constexpr bool PAIRED = kNarrow<T>;
+  static constexpr bool Q2 = kNarrow<T>;  // steps in pairs (march_tile)
   static constexpr bool DAMP = true;
   const T *__restrict__ s1;
   const T *__restrict__ r1;
@@ -162,7 +166,7 @@ struct FwdOp {
   }
   template <int VX>
   __device__ void prefetchv(const Blk2 &b, float (&p)[NPRE][VX], uint32_t o) const {
-    if constexpr (H) {
+    if constexpr (PAIRED) {
       float sr[2][VX];
       ld_pair<VX>(sr0, o, b.io, sr, nlev);
 #pragma unroll
@@ -187,7 +191,7 @@ struct FwdOp {
   }
   template <int VX>
   __device__ void storev(const Blk2 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
-    if constexpr (H) {
+    if constexpr (PAIRED) {
       st_pair<VX, NT>(srn, o, out[0], out[1], b.sn, nlev);
     } else {
       st_vec<VX, NT>(s_new, o, out[0], b.sn[0], nlev);
@@ -344,7 +348,7 @@ struct SrecOp {
   static constexpr bool H = kScaled<T>;
   static constexpr int NF = 1, NPRE = 2, NOUT = 2, NMAX = H ? 1 : 0;
   static constexpr bool PAIRED = false;
-  static constexpr bool Q2 = H;  // steps in pairs (march_tile)
+  static constexpr bool Q2 = kNarrow<T>;  // steps in pairs (march_tile)
   static constexpr bool DAMP = true;
   const T *__restrict__ s1;
   // old s level (read) and new s level (written): the same buffer, or with
@@ -395,7 +399,7 @@ struct SrecOp {
   __device__ void prefetchv(const Blk1 &b, float (&p)[NPRE][VX], uint32_t o) const {
     ld_stream<VX>(s0, o, b.io, p[0], nlev);
     // FP32: the velocity without the hint (114 -> 103 us per step at 304^3)
-    if constexpr (H)
+    if constexpr (kNarrow<T>)
       ld_stream<VX>(vel, o, 1.f, p[1]);
     else
       ld_vec<VX>(vel, o, 1.f, p[1]);
@@ -425,7 +429,7 @@ struct LamOp {
   static constexpr bool H = kScaled<T>;
   static constexpr int NF = 1, NPRE = 4, NOUT = 2, NMAX = H ? 1 : 0;
   static constexpr bool PAIRED = false;
-  static constexpr bool Q2 = H;  // steps in pairs (march_tile)
+  static constexpr bool Q2 = kNarrow<T>;  // steps in pairs (march_tile)
   static constexpr bool DAMP = true;
   const T *__restrict__ u1;
   T *__restrict__ u0;
@@ -477,7 +481,7 @@ struct LamOp {
     // FP32: velocity and image without the non-temporal hint (read, and the
     // image written, every step: 143 against 155 us per step at 304^3);
     // the FP16 step is faster with it
-    if constexpr (H) {
+    if constexpr (kNarrow<T>) {
       ld_stream<VX>(vel, o, 1.f, p[1]);
       ld_stream<VX>(acc, o, 1.f, p[3]);
     } else {
@@ -488,7 +492,7 @@ struct LamOp {
   template <int VX>
   __device__ void storev(const Blk1 &b, uint32_t o, const float (&out)[NOUT][VX]) const {
     st_vec<VX>(u0, o, out[0], b.sn, nlev);
-    st_vec<VX, H>(acc, o, out[1], 1.f);
+    st_vec<VX, kNarrow<T>>(acc, o, out[1], 1.f);
   }
   __device__ void cell(const Blk1 &, int slot, const CellArgs &c, const float *lap,
                        const float *ctr, const float *p, float (&out)[NOUT],

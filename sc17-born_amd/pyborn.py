@@ -253,8 +253,8 @@ class Propagator:
 
     def set_storage(self, kind):
         """Wavefield storage of the Born forward / exact adjoint: "f32",
-        "f16" (FP16 levels with per-level power-of-two scales) or "f24" (the
-        same scaled FP16 plus a refinement byte per cell)."""
+        "f16" (FP16 levels with per-level power-of-two scales) or "f24"
+        (FP32 values rounded to 24 bits: 16 significant bits, 3 bytes)."""
         self._chk(self.lib.born_set_storage(self.h, self.STORAGE[kind]), "storage")
 
     def set_step_pairs(self, mode):

@@ -1,12 +1,12 @@
 """24-bit wavefield storage (born_set_storage(BORN_STORAGE_F24)) on the GPU:
-every stored level is an FP16 value plus a refinement byte per cell, with
-the per-level power-of-two scales of FP16 storage; the arithmetic stays FP32.
+every stored level holds its FP32 values rounded to 24 bits (16 significant
+bits, the FP32 exponent, no scale); the arithmetic stays FP32.
 
   * data and exact-adjoint image against the FP64 oracle: rel-L2 <= 1e-3,
     the accuracy bar for the 16-bit-based storage, where FP16 levels alone
     miss it (their rounding noise grows with the step count);
-  * checkpointed background recompute restores the scale ring with the
-    levels: every plan gives bitwise the same image;
+  * checkpointed background recompute from 24-bit checkpoints: every plan
+    gives bitwise the same image;
   * the history kept by the forward gives the image of the recomputed one;
   * dot-product mismatch of the pair with white-noise inputs;
   * every shot of a batch gives the traces of the same shot run alone.
@@ -50,7 +50,7 @@ def test_f24_data_and_image_against_fp64(pyborn, scatter):
     print("scatter %d: f16 data %.2e image %.2e | f24 data %.2e image %.2e"
           % (scatter, *err["f16"], *err["f24"]))
     assert max(err["f24"]) <= F24_ERR
-    # the refinement byte takes the error well below FP16's
+    # 16 significant bits take the error well below FP16's 11
     assert max(err["f24"]) < 0.1 * max(err["f16"])
 
 

@@ -1,7 +1,7 @@
 """16-bit-based storage error on the 512x512x400 configuration's physics, on
 a 128^3 crop (the FP64 oracle's share): data, image and gradient of the Born
 pair against the FP64 oracle, 2500 steps at 0.8 ms, for FP32 levels, FP16
-levels (an approximate mode) and 24-bit levels (FP16 plus a refinement byte,
+levels (an approximate mode) and 24-bit levels (FP32 rounded to 24 bits,
 held to the 1e-3 bar).  The oracle takes a few minutes of CPU time.
 
   data     F m                    (m: smoothed noise, z-differenced)
