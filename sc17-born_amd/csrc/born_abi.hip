@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -111,6 +112,26 @@ struct born_ctx {
   int last_segments = 0, last_recomputed = 0, last_swept = 0;
   int last_pairs = 0;
   int last_lam_pairs = 0;
+
+  // Placement of a two-field work pair: r of the forward, nu of the
+  // adjoint state.  A step's launch time depends on where its streams sit
+  // in HBM relative to one another (the FP32 forward at 304^3: 157-178 us
+  // for the same kernel and shape, by 32 GiB windows of history offset), so
+  // a history of FP32 levels longer than one window keeps a gap of two
+  // fields before every window of seg_lpw levels, and the pair sits in the
+  // work buffers or in one of the gaps: the place timed fastest for the
+  // window the step's history levels are in.  place_mode 0 keeps the plain
+  // layout (no gaps), 1 times the places, 2 cycles through them (tests).
+  int place_mode = 1;
+  size_t place_window = (size_t)32 << 30;  // history bytes per window
+  size_t seg_gap = 0;     // bytes of a gap of d_seg (0: no gaps)
+  long long seg_lpw = 0;  // levels per window
+  int seg_ngaps = 0;
+  // place per window index (-1: the work buffers, g >= 0: gap g) and the
+  // buffers and launch shape it was chosen for
+  std::vector<int> place_fwd, place_lam;
+  std::vector<uintptr_t> place_fwd_key, place_lam_key;
+  int last_fwd_moves = 0, last_lam_moves = 0;
 
   // this device's scratch array of born_dev_source_prop_noalloc (resolved
   // at born_create), or the reason it could not be resolved
@@ -230,6 +251,8 @@ void reset_tuning(born_ctx *c) {
     c->tune[k] = is_pair(k) ? born::Tune{0, 128} : born::Tune{3, 64};
     c->tune_ms[k] = -1.f;
   }
+  c->place_fwd_key.clear();
+  c->place_lam_key.clear();
 }
 
 // A shape of two cells per lane needs an even nx.  For an odd nx (a shape
@@ -521,10 +544,46 @@ size_t history_budget(born_ctx *ctx) {
   return fr > reserve ? fr - reserve : 0;
 }
 
+// Layout of a segment history of L levels of esz bytes: FP32 levels of a
+// history longer than one window get a gap of two fields before every
+// window of lpw levels (the places of the work pair, see born_ctx).
+struct SegLayout {
+  size_t gap;     // bytes of a gap, 0: no gaps
+  long long lpw;  // levels per window
+  int ngaps;
+  size_t bytes;   // the whole history
+};
+SegLayout seg_layout(const born_ctx *ctx, size_t L, size_t esz) {
+  const size_t F = (size_t)ctx->n * esz;
+  SegLayout s{0, 0, 0, L * F};
+  if (ctx->place_mode == 0 || esz != 4 || F == 0) return s;
+  const long long lpw = std::max<long long>(1, (long long)(ctx->place_window / F));
+  if ((long long)L <= lpw) return s;
+  s.gap = 2 * F;
+  s.lpw = lpw;
+  s.ngaps = (int)(((long long)L - 1) / lpw + 1);
+  s.bytes += (size_t)s.ngaps * s.gap;
+  return s;
+}
+
+// level k of the segment history, in the layout of the last history_alloc
+float *seg_level(const born_ctx *ctx, long long k) {
+  size_t off = (size_t)k * ctx->n * ctx->seg_esz;
+  if (ctx->seg_gap) off += (size_t)(k / ctx->seg_lpw + 1) * ctx->seg_gap;
+  return (float *)((char *)ctx->d_seg + off);
+}
+
+// field i (0 or 1) of gap g of the segment history (FP32 levels)
+float *seg_gap_field(const born_ctx *ctx, int g, int i) {
+  const size_t w = (size_t)ctx->seg_lpw * ctx->n * ctx->seg_esz + ctx->seg_gap;
+  return (float *)((char *)ctx->d_seg + (size_t)g * w + (size_t)i * ctx->n * sizeof(float));
+}
+
 // history buffers: L levels of scatter operand + nck checkpoint levels, each
 // n elements of esz bytes (capacities are counted in floats)
 int history_alloc(born_ctx *ctx, size_t L, size_t nck, size_t esz) {
-  const size_t bytes = L * (size_t)ctx->n * esz;
+  const SegLayout lay = seg_layout(ctx, L, esz);
+  const size_t bytes = lay.bytes;
   const size_t seg = (bytes + 3) / 4;
   const size_t ck = (nck * (size_t)ctx->n * esz + 3) / 4;
   if (ctx->seg_cap < seg) {
@@ -535,15 +594,19 @@ int history_alloc(born_ctx *ctx, size_t L, size_t nck, size_t esz) {
     ctx->seg_cap = seg;
   }
   // shells of the levels about to be used: zero unless cleared for this
-  // layout (levels of another grid or element size leave values there)
+  // layout (levels of another grid, element size or gap layout leave
+  // values there)
   if (ctx->seg_nx != ctx->nx || ctx->seg_ny != ctx->ny || ctx->seg_nz != ctx->nz ||
-      ctx->seg_esz != esz) {
+      ctx->seg_esz != esz || ctx->seg_gap != lay.gap || ctx->seg_lpw != lay.lpw) {
     ctx->seg_clean = 0;
     ctx->seg_nx = ctx->nx;
     ctx->seg_ny = ctx->ny;
     ctx->seg_nz = ctx->nz;
     ctx->seg_esz = esz;
+    ctx->seg_gap = lay.gap;
+    ctx->seg_lpw = lay.lpw;
   }
+  ctx->seg_ngaps = lay.ngaps;
   if (ctx->seg_clean < bytes) {
     const size_t from = ctx->seg_clean / 4;
     CHK(zero(ctx, ctx->d_seg + from, (long long)(seg - from)));
@@ -558,15 +621,15 @@ int history_alloc(born_ctx *ctx, size_t L, size_t nck, size_t esz) {
 }
 
 // The checkpoint plan of an exact adjoint over nt steps: the fewest
-// segments S whose L = ceil(nt / S) levels of scatter operand plus 2 (S - 1)
-// checkpoint levels fit the history budget.
+// segments S whose L = ceil(nt / S) levels of scatter operand (with the
+// layout's gaps) plus 2 (S - 1) checkpoint levels fit the history budget.
 int checkpoint_plan(born_ctx *ctx, int nt, size_t esz, int *S, int *L) {
   const size_t F = (size_t)ctx->n * esz;
   const size_t avail = history_budget(ctx);
   int segs = 0;
   for (int sg = 1; sg <= nt; sg++) {
     const size_t l = (size_t)((nt + sg - 1) / sg);
-    if ((l + 2 * (size_t)(sg - 1)) * F <= avail) {
+    if (seg_layout(ctx, l, esz).bytes + 2 * (size_t)(sg - 1) * F <= avail) {
       segs = sg;
       break;
     }
@@ -616,6 +679,64 @@ size_t elem(const born_ctx *c) {
 // k-th field of element size esz in an array of fields of n cells
 float *nth(float *base, long long k, long long n, size_t esz) {
   return (float *)((char *)base + (size_t)k * n * esz);
+}
+
+// Moves the pair (a, b) of FP32 fields to place `to` (-1: the work buffers
+// w0, w1; g >= 0: gap g of the segment history): two field copies on the
+// stream, none when the pair is there already.
+int move_pair(born_ctx *ctx, int to, float *w0, float *w1, float *&a, float *&b) {
+  float *d0 = to < 0 ? w0 : seg_gap_field(ctx, to, 0);
+  float *d1 = to < 0 ? w1 : seg_gap_field(ctx, to, 1);
+  if ((a == d0 && b == d1) || (a == d1 && b == d0)) return BORN_OK;
+  const size_t F = (size_t)ctx->n * sizeof(float);
+  HIPCHK(hipMemcpyAsync(d0, a, F, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(d1, b, F, hipMemcpyDeviceToDevice, ctx->stream));
+  a = d0;
+  b = d1;
+  return BORN_OK;
+}
+
+// The place of the work pair for each of nwin window indices.  place_mode
+// 1: the fastest of the work buffers and the gaps over `reps` launches of
+// run(window, place, i), i = 1, 2, ... the launch's position in a march
+// (after an untimed i = 0); without autotuning, the work buffers.
+// place_mode 2: window w takes place w mod (gaps + 1) - 1, every place in
+// turn.  The result is kept while `key` (the buffers and launch shape it
+// was timed with) stays the same.
+template <class F>
+int place_tune(born_ctx *ctx, std::vector<int> &place, std::vector<uintptr_t> &held,
+               const std::vector<uintptr_t> &key, int nwin, F run) {
+  if (held == key && (int)place.size() == nwin) return BORN_OK;
+  place.assign(nwin, -1);
+  held.clear();
+  if (ctx->place_mode == 2) {
+    for (int w = 0; w < nwin; w++) place[w] = w % (ctx->seg_ngaps + 1) - 1;
+  } else if (ctx->autotune) {
+    constexpr int reps = 6;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int w = 0; w < nwin; w++) {
+      float best = 1e30f;
+      for (int p = -1; p < ctx->seg_ngaps; p++) {
+        HIPCHK(run(w, p, 0));
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+        for (int i = 1; i <= reps; i++) HIPCHK(run(w, p, i));
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          place[w] = p;
+        }
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  held = key;
+  return BORN_OK;
 }
 
 // ctx->d_ring (64 words): rings s, r at 0, 8 (scale[4], maxb[4]) and max
@@ -1063,7 +1184,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   // first.  (Scaled levels keep s and r together; their adjoint sweeps.)
   int ckS = 0, ckL = 0;
   if (keep_background && nt > 0) {
-    if ((size_t)nt * n * esz <= history_budget(ctx)) {
+    if (seg_layout(ctx, (size_t)nt, esz).bytes <= history_budget(ctx)) {
       CHK(history_alloc(ctx, (size_t)nt, 0, esz));
       hist = ctx->d_seg;
       if (half) CHK(ensure(ctx, &ctx->d_hscale, &ctx->hscale_cap, (size_t)nt));
@@ -1098,9 +1219,9 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     int lev = 0;
     CHK(autotune(ctx, kind, [&](const born::Tune &tu) {
       if (rotate) {
-        t.s0 = nth(hist, lev, n, esz);
-        t.s1 = nth(hist, lev + 1, n, esz);
-        t.s_new = nth(hist, lev + 2, n, esz);
+        t.s0 = seg_level(ctx, lev);
+        t.s1 = seg_level(ctx, lev + 1);
+        t.s_new = seg_level(ctx, lev + 2);
         lev = (lev + 1) % (nt - 2);
       }
       return born::launch_fwd_step(geo, tu, t, ctx->stream); }));
@@ -1122,6 +1243,50 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w2[i], n));
   }
   ctx->last_pairs = use_pairs;
+  // Where r sits, by the window of the history level a step writes: level
+  // it of a kept history, level it - klast in the last segment of a
+  // checkpoint plan, none (window index 0 then) before it.
+  const int klast = ckS > 1 ? (ckS - 1) * ckL : nt;
+  const bool place = ctx->place_mode && ctx->seg_gap && !half && !use_pairs && (hist || ckS > 1);
+  const int wbase = hist ? 0 : 1;
+  auto window_of = [&](int it) {
+    const long long lv = hist ? it : it >= klast ? it - klast : -1;
+    return lv < 0 ? 0 : (int)(lv / ctx->seg_lpw) + wbase;
+  };
+  if (place) {
+    const long long nlev = hist ? nt : ckL;
+    born::FwdStep t{};  // a step without sources or traces
+    t.vel = ctx->d_vel2; t.model = model; t.src = no_points(); t.rec = recmap(ctx);
+    t.tr = traces; t.tab = ctx->d_tab_d; t.extract = 0; t.scatter = scatter;
+    t.h = half_state(ctx, 0); t.rg = ctx->d_ring + RING_T0; t.hscale = ctx->d_ring + HS_T;
+    const std::vector<uintptr_t> key = {
+        (uintptr_t)ctx->d_seg, ctx->seg_gap, (uintptr_t)ctx->seg_lpw, (uintptr_t)nlev,
+        (uintptr_t)n, (uintptr_t)tu.shape, (uintptr_t)tu.zchunk, (uintptr_t)ctx->d_vel2,
+        (uintptr_t)model, (uintptr_t)scatter, (uintptr_t)wbase, (uintptr_t)ctx->d_w[2]};
+    CHK(place_tune(ctx, ctx->place_fwd, ctx->place_fwd_key, key, ctx->seg_ngaps + wbase,
+                   [&](int w, int p, int i) {
+      if (w < wbase) {  // s in the work buffers
+        t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.s_new = nullptr; t.hist = nullptr;
+      } else {  // three levels of the window, one on per launch, as in a march
+        const long long a = (long long)(w - wbase) * ctx->seg_lpw;
+        const long long b = std::min(nlev, a + ctx->seg_lpw);
+        const long long k = a + i % std::max(1LL, b - a - 2);
+        if (scatter == 0) {
+          t.s0 = seg_level(ctx, k);
+          t.s1 = seg_level(ctx, std::min(k + 1, b - 1));
+          t.s_new = seg_level(ctx, std::min(k + 2, b - 1));
+          t.hist = nullptr;
+        } else {
+          t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.s_new = nullptr;
+          t.hist = seg_level(ctx, k);
+        }
+      }
+      float *r0 = p < 0 ? ctx->d_w[2] : seg_gap_field(ctx, p, 0);
+      float *r1 = p < 0 ? ctx->d_w[3] : seg_gap_field(ctx, p, 1);
+      t.r0 = i % 2 ? r1 : r0;
+      t.r1 = i % 2 ? r0 : r1;
+      return born::launch_fwd_step(geo, tu, t, ctx->stream); }));
+  }
   for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w[i], (long long)ctx->w_cap));
   CHK(source_amps(ctx, nt, jt, 0, 1 / (float)jt));
   if (half) {
@@ -1147,8 +1312,8 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
       const int ida = it / jt, idb = (it + 1) / jt;
       p.s_old = sb[0]; p.s_cur = sb[1]; p.s_a = sb[2]; p.s_b = sb[3];
       p.r_old = rb[0]; p.r_cur = rb[1]; p.r_a = rb[2]; p.r_b = rb[3];
-      p.h_a = hist ? nth(hist, it, n, esz) : nullptr;
-      p.h_b = hist ? nth(hist, it + 1, n, esz) : nullptr;
+      p.h_a = hist ? seg_level(ctx, it) : nullptr;
+      p.h_b = hist ? seg_level(ctx, it + 1) : nullptr;
       p.amp_a = ctx->d_amp + (long long)it * ctx->npts_s;
       p.amp_b = ctx->d_amp + (long long)(it + 1) * ctx->npts_s;
       p.tab_a = ctx->d_tab_d + (long long)(it - ida * jt) * 8;
@@ -1186,10 +1351,18 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   // the segment history (scatter 0: the s levels themselves, level j > klast
   // in segment level j-1-klast, levels klast and klast-1 staying in s_cur
   // and s_old)
-  const int klast = ckS > 1 ? (ckS - 1) * ckL : nt;
   const size_t Fb = (size_t)n * esz;
+  int rplace = -1, moves = 0;  // where r is: the work buffers
   for (; it < nt; it++) {
     const int id = it / jt, ii = it - id * jt;
+    if (place) {
+      const int want = ctx->place_fwd[window_of(it)];
+      if (want != rplace) {
+        CHK(move_pair(ctx, want, ctx->d_w[2], ctx->d_w[3], r_old, r_cur));
+        rplace = want;
+        moves++;
+      }
+    }
     if (ckS > 1 && it > 0 && it % ckL == 0 && it / ckL <= ckS - 1) {
       const long long c = it / ckL - 1;
       HIPCHK(hipMemcpyAsync(nth(ctx->d_ckpt, 2 * c, n, esz), s_cur, Fb,
@@ -1199,26 +1372,25 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     }
     const bool into_seg = it >= klast;
     if (s_in_hist) {
-      a.s1 = it >= 1 ? nth(hist, it - 1, n, esz) : s_cur;  // level 0: zero
-      a.s0 = it >= 2 ? nth(hist, it - 2, n, esz) : s_old;
-      a.s_new = nth(hist, it, n, esz);
+      a.s1 = it >= 1 ? seg_level(ctx, it - 1) : s_cur;  // level 0: zero
+      a.s0 = it >= 2 ? seg_level(ctx, it - 2) : s_old;
+      a.s_new = seg_level(ctx, it);
       a.hist = nullptr;
     } else if (into_seg && scatter == 0) {
-      a.s1 = it > klast ? nth(ctx->d_seg, it - 1 - klast, n, esz) : s_cur;
-      a.s0 = it > klast + 1 ? nth(ctx->d_seg, it - 2 - klast, n, esz)
-                            : (it > klast ? s_cur : s_old);
-      a.s_new = nth(ctx->d_seg, it - klast, n, esz);
+      a.s1 = it > klast ? seg_level(ctx, it - 1 - klast) : s_cur;
+      a.s0 = it > klast + 1 ? seg_level(ctx, it - 2 - klast) : (it > klast ? s_cur : s_old);
+      a.s_new = seg_level(ctx, it - klast);
       a.hist = nullptr;
     } else if (into_seg) {
       a.s1 = s_cur;
       a.s0 = s_old;
       a.s_new = nullptr;
-      a.hist = nth(ctx->d_seg, it - klast, n, esz);
+      a.hist = seg_level(ctx, it - klast);
     } else {
       a.s1 = s_cur;
       a.s0 = s_old;
       a.s_new = nullptr;
-      a.hist = hist ? nth(hist, it, n, esz) : nullptr;
+      a.hist = hist ? seg_level(ctx, it) : nullptr;
     }
     a.r1 = r_cur;
     a.r0 = r_old;
@@ -1232,6 +1404,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     if (!(into_seg && scatter == 0)) std::swap(s_old, s_cur);
     std::swap(r_old, r_cur);
   }
+  ctx->last_fwd_moves = moves;
   if (hist) {
     ctx->hist_version = ctx->bg_version;
     ctx->hist_scatter = scatter;
@@ -1499,13 +1672,13 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
           HIPCHK(hipMemcpyAsync(ctx->d_ckring + 8 * c, ctx->d_ring + RING_S, 8 * sizeof(float),
                                 hipMemcpyDeviceToDevice, ctx->stream));
       }
-      float *sig = k >= klast ? nth(ctx->d_seg, k - klast, n, esz) : nullptr;
+      float *sig = k >= klast ? seg_level(ctx, k - klast) : nullptr;
       const bool into_seg = s_in_seg && k >= klast;
       if (into_seg) {
         // levels klast and klast-1 stay in s_cur and s_old; level j > klast
         // is in segment level j-1-klast
-        sr.s1 = k > klast ? nth(ctx->d_seg, k - 1 - klast, n, esz) : s_cur;
-        sr.s0 = k > klast + 1 ? nth(ctx->d_seg, k - 2 - klast, n, esz)
+        sr.s1 = k > klast ? seg_level(ctx, k - 1 - klast) : s_cur;
+        sr.s0 = k > klast + 1 ? seg_level(ctx, k - 2 - klast)
                               : (k > klast ? s_cur : s_old);
         sr.s_new = sig;
         sr.sig = nullptr;
@@ -1521,6 +1694,30 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
       HIPCHK(born::launch_srec_step(geo, stu, sr, ctx->stream));
       if (!into_seg) std::swap(s_old, s_cur);
     }
+  }
+  // where nu sits, by the window of the sigma level a step reads
+  const bool place = ctx->place_mode && ctx->seg_gap && !half && !lam_pairs;
+  if (place) {
+    born::LamStep t{};  // a step without receivers
+    t.vel = ctx->d_vel2; t.acc = ctx->d_acc; t.rec = recmap(ctx); t.tr = traces;
+    t.tab = ctx->d_tab_d; t.inject = 0;
+    t.h = half_state(ctx, 0); t.ring = ring(ctx, RING_T0); t.inj = inj;
+    t.trmax = maxp(ctx, MAX_T); t.hscale = ctx->d_ring + HS_T;
+    const std::vector<uintptr_t> key = {
+        (uintptr_t)ctx->d_seg, ctx->seg_gap, (uintptr_t)ctx->seg_lpw, (uintptr_t)L,
+        (uintptr_t)n, (uintptr_t)ltu.shape, (uintptr_t)ltu.zchunk, (uintptr_t)ctx->d_vel2,
+        (uintptr_t)ctx->d_acc, (uintptr_t)ctx->d_w[0]};
+    CHK(place_tune(ctx, ctx->place_lam, ctx->place_lam_key, key, ctx->seg_ngaps,
+                   [&](int w, int p, int i) {
+      // the sigma levels of the window, one on per launch
+      const long long a = (long long)w * ctx->seg_lpw;
+      const long long b = std::min<long long>(L, a + ctx->seg_lpw);
+      t.sig = seg_level(ctx, a + i % std::max(1LL, b - a));
+      float *u0 = p < 0 ? ctx->d_w[0] : seg_gap_field(ctx, p, 0);
+      float *u1 = p < 0 ? ctx->d_w[1] : seg_gap_field(ctx, p, 1);
+      t.u0 = i % 2 ? u1 : u0;
+      t.u1 = i % 2 ? u0 : u1;
+      return born::launch_lam_step(geo, ltu, t, ctx->stream); }));
   }
   // pass 2: adjoint state, segment by segment in reverse
   float *u_old = ctx->d_w[0], *u_cur = ctx->d_w[1];
@@ -1542,6 +1739,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   lm.inj = inj;
   lm.trmax = maxp(ctx, MAX_TR);
   int step = 0;  // adjoint steps taken (indexes the ring of nu)
+  int uplace = -1, umoves = 0;  // where nu is: the work buffers
   // nu levels j-1, j, j+1, j+2 rotate through four buffers in two-step launches
   float *ub[4] = {u_old, u_cur, ctx->d_w2[0], ctx->d_w2[1]};
   born::LamPairStep lp{};
@@ -1568,10 +1766,10 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
                                 hipMemcpyDeviceToDevice, ctx->stream));
       }
       for (int k = k0; k < k1; k++) {
-        float *sig = nth(ctx->d_seg, k - k0, n, esz);
+        float *sig = seg_level(ctx, k - k0);
         if (s_in_seg) {
-          sr.s1 = k > k0 ? nth(ctx->d_seg, k - 1 - k0, n, esz) : s_cur;
-          sr.s0 = k > k0 + 1 ? nth(ctx->d_seg, k - 2 - k0, n, esz) : (k > k0 ? s_cur : s_old);
+          sr.s1 = k > k0 ? seg_level(ctx, k - 1 - k0) : s_cur;
+          sr.s0 = k > k0 + 1 ? seg_level(ctx, k - 2 - k0) : (k > k0 ? s_cur : s_old);
           sr.s_new = sig;
           sr.sig = nullptr;
         } else {
@@ -1593,8 +1791,8 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
       if (lam_pairs && k - 1 >= k0) {
         const int idb = (k - 1) / jt, iib = k - 1 - idb * jt;
         lp.u_old = ub[0]; lp.u_cur = ub[1]; lp.u_a = ub[2]; lp.u_b = ub[3];
-        lp.sig_a = nth(ctx->d_seg, k - k0, n, esz);
-        lp.sig_b = nth(ctx->d_seg, k - 1 - k0, n, esz);
+        lp.sig_a = seg_level(ctx, k - k0);
+        lp.sig_b = seg_level(ctx, k - 1 - k0);
         lp.tab_a = ctx->d_tab_d + (long long)ii * 8;
         lp.tab_b = ctx->d_tab_d + (long long)iib * 8;
         lp.id_a = id;
@@ -1608,9 +1806,17 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
         k -= 2;
         continue;
       }
+      if (place) {
+        const int want = ctx->place_lam[(int)((k - k0) / ctx->seg_lpw)];
+        if (want != uplace) {
+          CHK(move_pair(ctx, want, ctx->d_w[0], ctx->d_w[1], ub[0], ub[1]));
+          uplace = want;
+          umoves++;
+        }
+      }
       lm.u1 = ub[1];
       lm.u0 = ub[0];
-      lm.sig = nth(ctx->d_seg, k - k0, n, esz);
+      lm.sig = seg_level(ctx, k - k0);
       lm.hscale = half ? ctx->d_hscale + (k - k0) : nullptr;
       lm.tab = ctx->d_tab_d + (long long)ii * 8;
       lm.id = id;
@@ -1626,6 +1832,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   ctx->last_segments = S;
   ctx->last_recomputed = recomputed;
   ctx->last_swept = !cached && !planned ? nt : 0;
+  ctx->last_lam_moves = umoves;
   return BORN_OK;
 }
 
@@ -1753,6 +1960,22 @@ int born_last_background_sweep(const born_ctx *ctx) {
 int born_set_zchunk(born_ctx *ctx, int zchunk) {
   if (!ctx || zchunk < 0) return BORN_ERR_ARG;
   ctx->zchunk = zchunk;
+  return BORN_OK;
+}
+
+int born_set_placement(born_ctx *ctx, int mode, long long window_bytes) {
+  if (!ctx || mode < 0 || mode > 2 || window_bytes < 0) return BORN_ERR_ARG;
+  ctx->place_mode = mode;
+  ctx->place_window = window_bytes ? (size_t)window_bytes : (size_t)32 << 30;
+  ctx->place_fwd_key.clear();
+  ctx->place_lam_key.clear();
+  return BORN_OK;
+}
+
+int born_last_placement(const born_ctx *ctx, int *forward_moves, int *adjoint_moves) {
+  if (!ctx) return BORN_ERR_ARG;
+  if (forward_moves) *forward_moves = ctx->last_fwd_moves;
+  if (adjoint_moves) *adjoint_moves = ctx->last_lam_moves;
   return BORN_OK;
 }
 

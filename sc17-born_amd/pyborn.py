@@ -59,6 +59,8 @@ SIGNATURES = {
     "born_set_zchunk": (_i, [_vp, _i]),
     "born_set_storage": (_i, [_vp, _i]),
     "born_set_step_pairs": (_i, [_vp, _i]),
+    "born_set_placement": (_i, [_vp, _i, C.c_longlong]),
+    "born_last_placement": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "born_last_step_pairs": (_i, [_vp]),
     "born_set_autotune": (_i, [_vp, _i]),
     "born_get_tuning": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
@@ -256,6 +258,23 @@ class Propagator:
         "f16" (FP16 levels with per-level power-of-two scales) or "f24"
         (FP32 values rounded to 24 bits: 16 significant bits, 3 bytes)."""
         self._chk(self.lib.born_set_storage(self.h, self.STORAGE[kind]), "storage")
+
+    PLACEMENT = {"off": 0, "timed": 1, "cycle": 2}
+
+    def set_placement(self, mode="timed", window_bytes=0):
+        """Where the two levels of r (forward) and of the adjoint state sit
+        in HBM for FP32 histories longer than one window: "timed" (default)
+        the place timed fastest per window of history levels, "cycle" every
+        place in turn, "off" the work buffers and a history without gaps;
+        window_bytes: history bytes per window (0: 32 GiB)."""
+        self._chk(self.lib.born_set_placement(self.h, self.PLACEMENT[mode], int(window_bytes)),
+                  "placement")
+
+    def last_placement(self):
+        """(forward, adjoint) moves of the level pair in the last calls"""
+        f, a = C.c_int(), C.c_int()
+        self._chk(self.lib.born_last_placement(self.h, C.byref(f), C.byref(a)), "placement")
+        return f.value, a.value
 
     def set_step_pairs(self, mode):
         """Two time steps per launch in the FP32 Born forward: 0 / False
