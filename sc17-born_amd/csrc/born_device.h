@@ -282,6 +282,32 @@ __device__ __forceinline__ uint32_t points_on_plane(const Points &pts,
   return m;
 }
 
+// The source points of a block: the mask of those inside and the range of
+// their z planes, so that the test of a plane off that range loads nothing
+// (a loop of dependent scalar loads per plane cost the block that holds a
+// 9-point source about 8 us of a 160 us step at 304^3).
+struct PtsBlk {
+  uint32_t m;
+  int zlo, zhi;
+};
+__device__ __forceinline__ PtsBlk points_of_block(const Points &pts, int x0, int x1, int y0,
+                                                  int y1, int z0, int z1) {
+  PtsBlk b{0u, 1, 0};
+  for (int i = 0; i < pts.n; i++) {
+    const int4 q = ldc(pts.p, i);
+    if (q.x >= x0 && q.x < x1 && q.y >= y0 && q.y < y1 && q.z >= z0 && q.z < z1) {
+      b.zlo = b.m ? min(b.zlo, q.z) : q.z;
+      b.zhi = b.m ? max(b.zhi, q.z) : q.z;
+      b.m |= 1u << i;
+    }
+  }
+  return b;
+}
+__device__ __forceinline__ uint32_t points_on_plane(const Points &pts, const PtsBlk &b,
+                                                    int z) {
+  return z < b.zlo || z > b.zhi ? 0u : points_on_plane(pts, b.m, z);
+}
+
 // v plus, in list order, amp[i] of the points at (x, y) among plane mask m
 __device__ __forceinline__ float add_points(const Points &pts, uint32_t m, int x,
                                             int y, float v, const float *amp) {

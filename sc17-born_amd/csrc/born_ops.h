@@ -60,11 +60,11 @@ template <class T>
 using PairOf = typename std::conditional<kSplit<T>, SplitPair, __half2>::type;
 
 struct Blk1 {  // one stored field
-  uint32_t pm;
+  PtsBlk pb;
   float ic, io, sn, sh;  // 1/scale of cur and old, scale of new and of sigma
 };
 struct Blk2 {  // two stored fields
-  uint32_t pm;
+  PtsBlk pb;
   float ic[2], io[2], sn[2], sh;
 };
 
@@ -122,7 +122,7 @@ struct FwdOp {
 
   __device__ Blk2 block(int x0, int x1, int y0, int y1, int z0, int z1) const {
     Blk2 b;
-    b.pm = points_in_block(pts, x0, x1, y0, y1, z0, z1);
+    b.pb = points_of_block(pts, x0, x1, y0, y1, z0, z1);
     for (int f = 0; f < 2; f++) b.ic[f] = b.io[f] = b.sn[f] = 1.f;
     b.sh = 1.f;
     if constexpr (H) {
@@ -162,7 +162,7 @@ struct FwdOp {
     ld_pair<VX>(sr1, o, b.ic, v, nlev);
   }
   __device__ SrcPln plane(const Blk2 &b, int z) const {
-    return {points_on_plane(pts, b.pm, z), ldc(rm.plane_slot, z)};
+    return {points_on_plane(pts, b.pb, z), ldc(rm.plane_slot, z)};
   }
   template <int VX>
   __device__ void prefetchv(const Blk2 &b, float (&p)[NPRE][VX], uint32_t o) const {
@@ -368,7 +368,7 @@ struct SrecOp {
 
   __device__ Blk1 block(int x0, int x1, int y0, int y1, int z0, int z1) const {
     Blk1 b;
-    b.pm = points_in_block(pts, x0, x1, y0, y1, z0, z1);
+    b.pb = points_of_block(pts, x0, x1, y0, y1, z0, z1);
     b.ic = b.io = b.sn = b.sh = 1.f;
     if constexpr (H) {
       b.ic = 1.f / ring.scale[sl.cur];
@@ -393,7 +393,7 @@ struct SrecOp {
     ld_vec<VX>(s1, o, b.ic, v, nlev);
   }
   __device__ uint32_t plane(const Blk1 &b, int z) const {
-    return points_on_plane(pts, b.pm, z);
+    return points_on_plane(pts, b.pb, z);
   }
   template <int VX>
   __device__ void prefetchv(const Blk1 &b, float (&p)[NPRE][VX], uint32_t o) const {
@@ -450,7 +450,7 @@ struct LamOp {
 
   __device__ Blk1 block(int, int, int, int, int, int) const {
     Blk1 b;
-    b.pm = 0;
+    b.pb = PtsBlk{0u, 1, 0};
     b.ic = b.io = b.sn = b.sh = 1.f;
     if constexpr (H) {
       b.ic = 1.f / ring.scale[sl.cur];
