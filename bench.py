@@ -501,7 +501,7 @@ def main():
     if not args.no_checkpointed and not args.budget_gb:
         # a budget of half the FP32 history: two segments, the second half of
         # the background recomputed once from a checkpoint
-        half_gb = nt * p.nx * p.ny * p.nz * 4 / 2 ** 30 / 2 + 1
+        half_gb = nt * p.nx * p.ny * p.nz * 4 / 2 ** 30 / 2 + 2
         elc, fec, aec = run_leg("f32", leg_steps, leg_warm, int(half_gb * 2 ** 30))
         ckpt = (elc, fec, aec, P.last_checkpoint_plan(), half_gb, P.last_background_sweep())
         P.set_checkpoint_budget(0)

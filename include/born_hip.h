@@ -205,15 +205,17 @@ int born_last_step_pairs(const born_ctx *ctx);
  * adjoint state's two levels (exact adjoint) in HBM, for FP32 histories
  * longer than one window.  A step kernel's speed depends on where the
  * arrays it streams sit relative to one another (up to 13 % at 304^3, by
- * 32 GiB windows of history offset), so such a history keeps a gap of two
- * fields before every window, and the pair of levels moves (two field
- * copies) between the work buffers and the gaps as the history levels a
- * step touches change window.  mode 1 (default): each window takes the
- * place timed fastest there (once per grid, launch shape and buffers; the
- * work buffers when the autotuner is off); 2: the places in turn, window
- * after window; 0: no gaps and no moves.  window_bytes: history bytes per
- * window (0: 32 GiB).  Results do not depend on either.  No reference
- * counterpart: the reference holds no history on a GPU. */
+ * 32 GiB windows of history offset), so such a history is laid out in
+ * windows at fixed offsets of its buffer, each a gap of two fields and
+ * then levels, and the pair of levels moves (two field copies) from gap to
+ * gap as the history levels a step touches change window.  mode 1
+ * (default): each window takes the gap timed fastest for it (timed once
+ * per grid, history buffer and the buffers of velocity and model; the work
+ * buffers when the autotuner is off); 2: the work buffers and the gaps in turn,
+ * window after window; 0: no gaps and no moves.  window_bytes: bytes per
+ * window (0: 32 GiB; at least three fields).  Results do not depend on
+ * either.  No reference counterpart: the reference holds no history on a
+ * GPU. */
 int born_set_placement(born_ctx *ctx, int mode, long long window_bytes);
 /* pair moves of the last Born forward and of the last exact adjoint (the
  * first move, from the work buffers, counts) */

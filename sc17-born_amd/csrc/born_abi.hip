@@ -117,18 +117,21 @@ struct born_ctx {
   // adjoint state.  A step's launch time depends on where its streams sit
   // in HBM relative to one another (the FP32 forward at 304^3: 157-178 us
   // for the same kernel and shape, by 32 GiB windows of history offset), so
-  // a history of FP32 levels longer than one window keeps a gap of two
-  // fields before every window of seg_lpw levels, and the pair sits in the
-  // work buffers or in one of the gaps: the place timed fastest for the
-  // window the step's history levels are in.  place_mode 0 keeps the plain
-  // layout (no gaps), 1 times the places, 2 cycles through them (tests).
+  // a history of FP32 levels longer than one window is laid out in windows
+  // of seg_win bytes at fixed offsets of d_seg, each a gap of two fields
+  // followed by seg_lpw levels, and the pair sits in the gap timed fastest
+  // for the window of the history levels a step touches.  place_mode 0
+  // keeps the plain layout (no gaps), 1 times the places, 2 cycles through
+  // them (tests).
   int place_mode = 1;
   size_t place_window = (size_t)32 << 30;  // history bytes per window
   size_t seg_gap = 0;     // bytes of a gap of d_seg (0: no gaps)
+  size_t seg_win = 0;     // bytes per window
   long long seg_lpw = 0;  // levels per window
   int seg_ngaps = 0;
-  // place per window index (-1: the work buffers, g >= 0: gap g) and the
-  // buffers and launch shape it was chosen for
+  // the gap timed fastest per window (-1: not timed) and what it was timed
+  // for: the history buffer and window size, the grid, the other arrays the
+  // step streams (their places matter too) and the launch shape
   std::vector<int> place_fwd, place_lam;
   std::vector<uintptr_t> place_fwd_key, place_lam_key;
   int last_fwd_moves = 0, last_lam_moves = 0;
@@ -545,38 +548,44 @@ size_t history_budget(born_ctx *ctx) {
 }
 
 // Layout of a segment history of L levels of esz bytes: FP32 levels of a
-// history longer than one window get a gap of two fields before every
-// window of lpw levels (the places of the work pair, see born_ctx).
+// history longer than one window are laid out in windows of W bytes at
+// offsets 0, W, 2W, ...: a gap of two fields (the places of the work pair,
+// see born_ctx), then lpw levels.
 struct SegLayout {
   size_t gap;     // bytes of a gap, 0: no gaps
+  size_t win;     // bytes per window
   long long lpw;  // levels per window
   int ngaps;
   size_t bytes;   // the whole history
 };
 SegLayout seg_layout(const born_ctx *ctx, size_t L, size_t esz) {
   const size_t F = (size_t)ctx->n * esz;
-  SegLayout s{0, 0, 0, L * F};
-  if (ctx->place_mode == 0 || esz != 4 || F == 0) return s;
-  const long long lpw = std::max<long long>(1, (long long)(ctx->place_window / F));
+  SegLayout s{0, 0, 0, 0, L * F};
+  const size_t W = ctx->place_window;
+  if (ctx->place_mode == 0 || esz != 4 || F == 0 || W < 3 * F) return s;
+  const long long lpw = (long long)((W - 2 * F) / F);
   if ((long long)L <= lpw) return s;
   s.gap = 2 * F;
+  s.win = W;
   s.lpw = lpw;
   s.ngaps = (int)(((long long)L - 1) / lpw + 1);
-  s.bytes += (size_t)s.ngaps * s.gap;
+  s.bytes = (size_t)(s.ngaps - 1) * W + s.gap + ((long long)L - (s.ngaps - 1) * lpw) * F;
   return s;
 }
 
 // level k of the segment history, in the layout of the last history_alloc
 float *seg_level(const born_ctx *ctx, long long k) {
-  size_t off = (size_t)k * ctx->n * ctx->seg_esz;
-  if (ctx->seg_gap) off += (size_t)(k / ctx->seg_lpw + 1) * ctx->seg_gap;
+  const size_t F = (size_t)ctx->n * ctx->seg_esz;
+  const size_t off = ctx->seg_gap ? (size_t)(k / ctx->seg_lpw) * ctx->seg_win + ctx->seg_gap +
+                                        (size_t)(k % ctx->seg_lpw) * F
+                                  : (size_t)k * F;
   return (float *)((char *)ctx->d_seg + off);
 }
 
 // field i (0 or 1) of gap g of the segment history (FP32 levels)
 float *seg_gap_field(const born_ctx *ctx, int g, int i) {
-  const size_t w = (size_t)ctx->seg_lpw * ctx->n * ctx->seg_esz + ctx->seg_gap;
-  return (float *)((char *)ctx->d_seg + (size_t)g * w + (size_t)i * ctx->n * sizeof(float));
+  return (float *)((char *)ctx->d_seg + (size_t)g * ctx->seg_win +
+                   (size_t)i * ctx->n * sizeof(float));
 }
 
 // history buffers: L levels of scatter operand + nck checkpoint levels, each
@@ -597,13 +606,14 @@ int history_alloc(born_ctx *ctx, size_t L, size_t nck, size_t esz) {
   // layout (levels of another grid, element size or gap layout leave
   // values there)
   if (ctx->seg_nx != ctx->nx || ctx->seg_ny != ctx->ny || ctx->seg_nz != ctx->nz ||
-      ctx->seg_esz != esz || ctx->seg_gap != lay.gap || ctx->seg_lpw != lay.lpw) {
+      ctx->seg_esz != esz || ctx->seg_gap != lay.gap || ctx->seg_win != lay.win) {
     ctx->seg_clean = 0;
     ctx->seg_nx = ctx->nx;
     ctx->seg_ny = ctx->ny;
     ctx->seg_nz = ctx->nz;
     ctx->seg_esz = esz;
     ctx->seg_gap = lay.gap;
+    ctx->seg_win = lay.win;
     ctx->seg_lpw = lay.lpw;
   }
   ctx->seg_ngaps = lay.ngaps;
@@ -696,47 +706,89 @@ int move_pair(born_ctx *ctx, int to, float *w0, float *w1, float *&a, float *&b)
   return BORN_OK;
 }
 
-// The place of the work pair for each of nwin window indices.  place_mode
-// 1: the fastest of the work buffers and the gaps over `reps` launches of
-// run(window, place, i), i = 1, 2, ... the launch's position in a march
-// (after an untimed i = 0); without autotuning, the work buffers.
+// The place of the work pair in each of the nwin windows of the current
+// layout, into use[w] (-1: the work buffers, g >= 0: gap g).  place_mode 1:
+// the gap timed fastest over 4 launches of run(window, gap, i), i = 1, 2, ...
+// the launch's position in a march (after an untimed i = 0); a choice is
+// kept in `cache` and timed again only when `key` changes or the gap it
+// names is past this layout's; without autotuning, the work buffers.
+// The differences it finds are 5-13 % and come in classes: the gaps that
+// time within 2 % of each other on the first window timed give the same
+// time on every window (they sit in the same kind of HBM window), so that
+// window times every gap and the later ones one gap of each class, the
+// fastest of the class on the first.
 // place_mode 2: window w takes place w mod (gaps + 1) - 1, every place in
-// turn.  The result is kept while `key` (the buffers and launch shape it
-// was timed with) stays the same.
+// turn.
 template <class F>
-int place_tune(born_ctx *ctx, std::vector<int> &place, std::vector<uintptr_t> &held,
-               const std::vector<uintptr_t> &key, int nwin, F run) {
-  if (held == key && (int)place.size() == nwin) return BORN_OK;
-  place.assign(nwin, -1);
-  held.clear();
+int place_plan(born_ctx *ctx, std::vector<int> &cache, std::vector<uintptr_t> &held,
+               const std::vector<uintptr_t> &key, int nwin, std::vector<int> &use, F run) {
+  const int ng = ctx->seg_ngaps;
+  use.assign(nwin, -1);
   if (ctx->place_mode == 2) {
-    for (int w = 0; w < nwin; w++) place[w] = w % (ctx->seg_ngaps + 1) - 1;
-  } else if (ctx->autotune) {
-    constexpr int reps = 6;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    for (int w = 0; w < nwin; w++) {
-      float best = 1e30f;
-      for (int p = -1; p < ctx->seg_ngaps; p++) {
-        HIPCHK(run(w, p, 0));
-        HIPCHK(hipEventRecord(e0, ctx->stream));
-        for (int i = 1; i <= reps; i++) HIPCHK(run(w, p, i));
-        HIPCHK(hipEventRecord(e1, ctx->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-          best = ms;
-          place[w] = p;
+    for (int w = 0; w < nwin; w++) use[w] = w % (ng + 1) - 1;
+    return BORN_OK;
+  }
+  if (!ctx->autotune) return BORN_OK;
+  if (held != key) {
+    cache.clear();
+    held = key;
+  }
+  if ((int)cache.size() < nwin) cache.resize(nwin, -1);
+  constexpr int reps = 2;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto time_gap = [&](int w, int g, float *ms) -> int {
+    if (!e0) {
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+    }
+    HIPCHK(run(w, g, 0));
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int i = 1; i <= reps; i++) HIPCHK(run(w, g, i));
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(ms, e0, e1));
+    return BORN_OK;
+  };
+  std::vector<int> classes;  // one gap per class, after the first timed window
+  int rc = BORN_OK;
+  for (int w = 0; w < nwin && rc == BORN_OK; w++) {
+    if (cache[w] < 0 || cache[w] >= ng) {
+      if (classes.empty()) {
+        std::vector<std::pair<float, int>> t(ng);
+        for (int g = 0; g < ng && rc == BORN_OK; g++) {
+          t[g].second = g;
+          rc = time_gap(w, g, &t[g].first);
+        }
+        if (rc != BORN_OK) break;
+        std::sort(t.begin(), t.end());
+        classes.push_back(t[0].second);
+        for (int k = 1; k < ng; k++)
+          if (t[k].first > 1.02f * t[k - 1].first) classes.push_back(t[k].second);
+        cache[w] = t[0].second;
+      } else {
+        float best = 1e30f;
+        for (int g : classes) {
+          float ms = 0.f;
+          rc = time_gap(w, g, &ms);
+          if (rc != BORN_OK) break;
+          if (ms < best) {
+            best = ms;
+            cache[w] = g;
+          }
         }
       }
     }
+    use[w] = cache[w];
+  }
+  if (e0) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
-  held = key;
-  return BORN_OK;
+  if (rc != BORN_OK) {
+    cache.clear();
+    use.assign(nwin, -1);
+  }
+  return rc;
 }
 
 // ctx->d_ring (64 words): rings s, r at 0, 8 (scale[4], maxb[4]) and max
@@ -1245,14 +1297,15 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   ctx->last_pairs = use_pairs;
   // Where r sits, by the window of the history level a step writes: level
   // it of a kept history, level it - klast in the last segment of a
-  // checkpoint plan, none (window index 0 then) before it.
+  // checkpoint plan; before klast (s in the work buffers) r stays in the
+  // work buffers.
   const int klast = ckS > 1 ? (ckS - 1) * ckL : nt;
   const bool place = ctx->place_mode && ctx->seg_gap && !half && !use_pairs && (hist || ckS > 1);
-  const int wbase = hist ? 0 : 1;
   auto window_of = [&](int it) {
     const long long lv = hist ? it : it >= klast ? it - klast : -1;
-    return lv < 0 ? 0 : (int)(lv / ctx->seg_lpw) + wbase;
+    return lv < 0 ? -1 : (int)(lv / ctx->seg_lpw);
   };
+  std::vector<int> rplaces;
   if (place) {
     const long long nlev = hist ? nt : ckL;
     born::FwdStep t{};  // a step without sources or traces
@@ -1260,31 +1313,25 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     t.tr = traces; t.tab = ctx->d_tab_d; t.extract = 0; t.scatter = scatter;
     t.h = half_state(ctx, 0); t.rg = ctx->d_ring + RING_T0; t.hscale = ctx->d_ring + HS_T;
     const std::vector<uintptr_t> key = {
-        (uintptr_t)ctx->d_seg, ctx->seg_gap, (uintptr_t)ctx->seg_lpw, (uintptr_t)nlev,
-        (uintptr_t)n, (uintptr_t)tu.shape, (uintptr_t)tu.zchunk, (uintptr_t)ctx->d_vel2,
-        (uintptr_t)model, (uintptr_t)scatter, (uintptr_t)wbase, (uintptr_t)ctx->d_w[2]};
-    CHK(place_tune(ctx, ctx->place_fwd, ctx->place_fwd_key, key, ctx->seg_ngaps + wbase,
-                   [&](int w, int p, int i) {
-      if (w < wbase) {  // s in the work buffers
-        t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.s_new = nullptr; t.hist = nullptr;
-      } else {  // three levels of the window, one on per launch, as in a march
-        const long long a = (long long)(w - wbase) * ctx->seg_lpw;
-        const long long b = std::min(nlev, a + ctx->seg_lpw);
-        const long long k = a + i % std::max(1LL, b - a - 2);
-        if (scatter == 0) {
-          t.s0 = seg_level(ctx, k);
-          t.s1 = seg_level(ctx, std::min(k + 1, b - 1));
-          t.s_new = seg_level(ctx, std::min(k + 2, b - 1));
-          t.hist = nullptr;
-        } else {
-          t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.s_new = nullptr;
-          t.hist = seg_level(ctx, k);
-        }
+        (uintptr_t)ctx->d_seg, ctx->seg_win, (uintptr_t)scatter, (uintptr_t)n,
+        (uintptr_t)ctx->d_vel2, (uintptr_t)model, (uintptr_t)tu.shape, (uintptr_t)tu.zchunk};
+    CHK(place_plan(ctx, ctx->place_fwd, ctx->place_fwd_key, key, ctx->seg_ngaps, rplaces,
+                   [&](int w, int g, int i) {
+      // three levels of the window, one on per launch, as in a march
+      const long long a = (long long)w * ctx->seg_lpw;
+      const long long b = std::min(nlev, a + ctx->seg_lpw);
+      const long long k = a + i % std::max(1LL, b - a - 2);
+      if (scatter == 0) {
+        t.s0 = seg_level(ctx, k);
+        t.s1 = seg_level(ctx, std::min(k + 1, b - 1));
+        t.s_new = seg_level(ctx, std::min(k + 2, b - 1));
+        t.hist = nullptr;
+      } else {
+        t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.s_new = nullptr;
+        t.hist = seg_level(ctx, k);
       }
-      float *r0 = p < 0 ? ctx->d_w[2] : seg_gap_field(ctx, p, 0);
-      float *r1 = p < 0 ? ctx->d_w[3] : seg_gap_field(ctx, p, 1);
-      t.r0 = i % 2 ? r1 : r0;
-      t.r1 = i % 2 ? r0 : r1;
+      t.r0 = seg_gap_field(ctx, g, i % 2);
+      t.r1 = seg_gap_field(ctx, g, 1 - i % 2);
       return born::launch_fwd_step(geo, tu, t, ctx->stream); }));
   }
   for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w[i], (long long)ctx->w_cap));
@@ -1356,7 +1403,8 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   for (; it < nt; it++) {
     const int id = it / jt, ii = it - id * jt;
     if (place) {
-      const int want = ctx->place_fwd[window_of(it)];
+      const int w = window_of(it);
+      const int want = w < 0 ? -1 : rplaces[w];
       if (want != rplace) {
         CHK(move_pair(ctx, want, ctx->d_w[2], ctx->d_w[3], r_old, r_cur));
         rplace = want;
@@ -1697,6 +1745,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   }
   // where nu sits, by the window of the sigma level a step reads
   const bool place = ctx->place_mode && ctx->seg_gap && !half && !lam_pairs;
+  std::vector<int> uplaces;
   if (place) {
     born::LamStep t{};  // a step without receivers
     t.vel = ctx->d_vel2; t.acc = ctx->d_acc; t.rec = recmap(ctx); t.tr = traces;
@@ -1704,19 +1753,16 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
     t.h = half_state(ctx, 0); t.ring = ring(ctx, RING_T0); t.inj = inj;
     t.trmax = maxp(ctx, MAX_T); t.hscale = ctx->d_ring + HS_T;
     const std::vector<uintptr_t> key = {
-        (uintptr_t)ctx->d_seg, ctx->seg_gap, (uintptr_t)ctx->seg_lpw, (uintptr_t)L,
-        (uintptr_t)n, (uintptr_t)ltu.shape, (uintptr_t)ltu.zchunk, (uintptr_t)ctx->d_vel2,
-        (uintptr_t)ctx->d_acc, (uintptr_t)ctx->d_w[0]};
-    CHK(place_tune(ctx, ctx->place_lam, ctx->place_lam_key, key, ctx->seg_ngaps,
-                   [&](int w, int p, int i) {
+        (uintptr_t)ctx->d_seg, ctx->seg_win, (uintptr_t)n, (uintptr_t)ctx->d_vel2,
+        (uintptr_t)ctx->d_acc, (uintptr_t)ltu.shape, (uintptr_t)ltu.zchunk};
+    CHK(place_plan(ctx, ctx->place_lam, ctx->place_lam_key, key, ctx->seg_ngaps, uplaces,
+                   [&](int w, int g, int i) {
       // the sigma levels of the window, one on per launch
       const long long a = (long long)w * ctx->seg_lpw;
       const long long b = std::min<long long>(L, a + ctx->seg_lpw);
       t.sig = seg_level(ctx, a + i % std::max(1LL, b - a));
-      float *u0 = p < 0 ? ctx->d_w[0] : seg_gap_field(ctx, p, 0);
-      float *u1 = p < 0 ? ctx->d_w[1] : seg_gap_field(ctx, p, 1);
-      t.u0 = i % 2 ? u1 : u0;
-      t.u1 = i % 2 ? u0 : u1;
+      t.u0 = seg_gap_field(ctx, g, i % 2);
+      t.u1 = seg_gap_field(ctx, g, 1 - i % 2);
       return born::launch_lam_step(geo, ltu, t, ctx->stream); }));
   }
   // pass 2: adjoint state, segment by segment in reverse
@@ -1807,7 +1853,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
         continue;
       }
       if (place) {
-        const int want = ctx->place_lam[(int)((k - k0) / ctx->seg_lpw)];
+        const int want = uplaces[(int)((k - k0) / ctx->seg_lpw)];
         if (want != uplace) {
           CHK(move_pair(ctx, want, ctx->d_w[0], ctx->d_w[1], ub[0], ub[1]));
           uplace = want;

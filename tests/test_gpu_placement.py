@@ -40,12 +40,13 @@ def test_pair_placement_leaves_results_unchanged(pyborn, scatter):
     want_tr, want = _pair(H, c, torch, model, w, scatter)
     assert H.last_placement() == (0, 0)
     assert np.abs(want).max() > 0 and np.abs(want_tr).max() > 0
-    # 8-level windows: 9 windows and 9 gaps; window w takes place w - 1
-    # (the work buffers first), so the pair moves at every window but the
-    # forward's first
+    # windows of 8 fields: a gap of two, then 6 levels; 12 windows.  Window
+    # w takes place w - 1 (the work buffers first), so the pair moves at
+    # every window but the forward's first
     H.set_placement("cycle", 8 * F)
     tr, img = _pair(H, c, torch, model, w, scatter)
-    assert H.last_placement() == (8, 9)
+    nwin = -(-c.nt // 6)
+    assert H.last_placement() == (nwin - 1, nwin)
     assert np.array_equal(tr, want_tr) and np.array_equal(img, want)
     # the timed choice (whatever it picks)
     H.set_placement("timed", 8 * F)
