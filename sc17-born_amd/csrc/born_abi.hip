@@ -562,10 +562,13 @@ SegLayout seg_layout(const born_ctx *ctx, size_t L, size_t esz) {
   const size_t F = (size_t)ctx->n * esz;
   SegLayout s{0, 0, 0, 0, L * F};
   const size_t W = ctx->place_window;
-  if (ctx->place_mode == 0 || esz != 4 || F == 0 || W < 3 * F) return s;
-  const long long lpw = (long long)((W - 2 * F) / F);
+  // a gap holds two levels of the work pair: FP32 r levels, or the s and r
+  // levels that narrow storage keeps together
+  const size_t G = 2 * (esz == 4 ? F : 2 * F);
+  if (ctx->place_mode == 0 || F == 0 || W < G + F) return s;
+  const long long lpw = (long long)((W - G) / F);
   if ((long long)L <= lpw) return s;
-  s.gap = 2 * F;
+  s.gap = G;
   s.win = W;
   s.lpw = lpw;
   s.ngaps = (int)(((long long)L - 1) / lpw + 1);
@@ -582,10 +585,9 @@ float *seg_level(const born_ctx *ctx, long long k) {
   return (float *)((char *)ctx->d_seg + off);
 }
 
-// field i (0 or 1) of gap g of the segment history (FP32 levels)
+// level i (0 or 1) of the work pair in gap g of the segment history
 float *seg_gap_field(const born_ctx *ctx, int g, int i) {
-  return (float *)((char *)ctx->d_seg + (size_t)g * ctx->seg_win +
-                   (size_t)i * ctx->n * sizeof(float));
+  return (float *)((char *)ctx->d_seg + (size_t)g * ctx->seg_win + (size_t)i * ctx->seg_gap / 2);
 }
 
 // history buffers: L levels of scatter operand + nck checkpoint levels, each
@@ -691,16 +693,15 @@ float *nth(float *base, long long k, long long n, size_t esz) {
   return (float *)((char *)base + (size_t)k * n * esz);
 }
 
-// Moves the pair (a, b) of FP32 fields to place `to` (-1: the work buffers
-// w0, w1; g >= 0: gap g of the segment history): two field copies on the
-// stream, none when the pair is there already.
-int move_pair(born_ctx *ctx, int to, float *w0, float *w1, float *&a, float *&b) {
+// Moves the pair (a, b) of levels of `bytes` each to place `to` (-1: the
+// work buffers w0, w1; g >= 0: gap g of the segment history): two copies on
+// the stream, none when the pair is there already.
+int move_pair(born_ctx *ctx, int to, float *w0, float *w1, float *&a, float *&b, size_t bytes) {
   float *d0 = to < 0 ? w0 : seg_gap_field(ctx, to, 0);
   float *d1 = to < 0 ? w1 : seg_gap_field(ctx, to, 1);
   if ((a == d0 && b == d1) || (a == d1 && b == d0)) return BORN_OK;
-  const size_t F = (size_t)ctx->n * sizeof(float);
-  HIPCHK(hipMemcpyAsync(d0, a, F, hipMemcpyDeviceToDevice, ctx->stream));
-  HIPCHK(hipMemcpyAsync(d1, b, F, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(d0, a, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(d1, b, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   a = d0;
   b = d1;
   return BORN_OK;
@@ -1295,12 +1296,12 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w2[i], n));
   }
   ctx->last_pairs = use_pairs;
-  // Where r sits, by the window of the history level a step writes: level
-  // it of a kept history, level it - klast in the last segment of a
-  // checkpoint plan; before klast (s in the work buffers) r stays in the
-  // work buffers.
+  // Where the work pair sits (r, or with narrow storage s and r together),
+  // by the window of the history level a step writes: level it of a kept
+  // history, level it - klast in the last segment of a checkpoint plan;
+  // before klast (s in the work buffers) r stays in the work buffers.
   const int klast = ckS > 1 ? (ckS - 1) * ckL : nt;
-  const bool place = ctx->place_mode && ctx->seg_gap && !half && !use_pairs && (hist || ckS > 1);
+  const bool place = ctx->place_mode && ctx->seg_gap && !use_pairs && (hist || ckS > 1);
   auto window_of = [&](int it) {
     const long long lv = hist ? it : it >= klast ? it - klast : -1;
     return lv < 0 ? -1 : (int)(lv / ctx->seg_lpw);
@@ -1321,6 +1322,13 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
       const long long a = (long long)w * ctx->seg_lpw;
       const long long b = std::min(nlev, a + ctx->seg_lpw);
       const long long k = a + i % std::max(1LL, b - a - 2);
+      if (half) {  // s and r together in the gap, the scatter operand apart
+        t.s0 = seg_gap_field(ctx, g, i % 2);
+        t.s1 = seg_gap_field(ctx, g, 1 - i % 2);
+        t.s_new = nullptr;
+        t.hist = seg_level(ctx, k);
+        return born::launch_fwd_step(geo, tu, t, ctx->stream);
+      }
       if (scatter == 0) {
         t.s0 = seg_level(ctx, k);
         t.s1 = seg_level(ctx, std::min(k + 1, b - 1));
@@ -1406,7 +1414,10 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
       const int w = window_of(it);
       const int want = w < 0 ? -1 : rplaces[w];
       if (want != rplace) {
-        CHK(move_pair(ctx, want, ctx->d_w[2], ctx->d_w[3], r_old, r_cur));
+        if (half)
+          CHK(move_pair(ctx, want, ctx->d_w[0], ctx->d_w[1], s_old, s_cur, 2 * Fb));
+        else
+          CHK(move_pair(ctx, want, ctx->d_w[2], ctx->d_w[3], r_old, r_cur, Fb));
         rplace = want;
         moves++;
       }
@@ -1744,7 +1755,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
     }
   }
   // where nu sits, by the window of the sigma level a step reads
-  const bool place = ctx->place_mode && ctx->seg_gap && !half && !lam_pairs;
+  const bool place = ctx->place_mode && ctx->seg_gap && !lam_pairs;
   std::vector<int> uplaces;
   if (place) {
     born::LamStep t{};  // a step without receivers
@@ -1855,7 +1866,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
       if (place) {
         const int want = uplaces[(int)((k - k0) / ctx->seg_lpw)];
         if (want != uplace) {
-          CHK(move_pair(ctx, want, ctx->d_w[0], ctx->d_w[1], ub[0], ub[1]));
+          CHK(move_pair(ctx, want, ctx->d_w[0], ctx->d_w[1], ub[0], ub[1], Fb));
           uplace = want;
           umoves++;
         }

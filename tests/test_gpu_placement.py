@@ -77,6 +77,41 @@ def test_pair_placement_leaves_results_unchanged(pyborn, scatter):
     H.close()
 
 
+@pytest.mark.parametrize("storage", ["f16", "f24"])
+def test_narrow_pair_placement_leaves_results_unchanged(pyborn, storage):
+    # narrow storage keeps s and r together in the work pair and writes the
+    # scatter operand apart: that pair moves through the gaps instead of r
+    torch = pytest.importorskip("torch")
+    c = make_case(48, 40, 32, nt=70, nbound=8, seed=12)
+    dev = torch.device("cuda:0")
+    model = torch.from_numpy(c.model.ravel().copy()).to(dev)
+    w = torch.from_numpy(np.random.default_rng(6).standard_normal((c.ntb, c.nrec))
+                         .astype(np.float32)).to(dev)
+    H = hip_prop(pyborn, c)
+    H.transfer_receiver_func(c.rec_nx, c.rec_ny, c.rec_locs, np.zeros((c.nrec, c.ntb), np.float32))
+    H.set_storage(storage)
+    F = c.nx * c.ny * c.nz * 4
+    H.set_placement("off")
+    want_tr, want = _pair(H, c, torch, model, w, 0)
+    assert H.last_placement() == (0, 0)
+    assert np.abs(want).max() > 0 and np.abs(want_tr).max() > 0
+    for mode in ("cycle", "timed"):
+        H.set_placement(mode, 8 * F)
+        tr, img = _pair(H, c, torch, model, w, 0)
+        fm, am = H.last_placement()
+        if mode == "cycle":
+            assert fm >= 2 and am >= 2
+        assert np.array_equal(tr, want_tr) and np.array_equal(img, want)
+    # an adjoint that rebuilds the background into a gapped history itself
+    H.set_checkpoint_budget(40 * F)
+    H.set_placement("cycle", 8 * F)
+    tr, img = _pair(H, c, torch, model, w, 0)
+    segs, _ = H.last_checkpoint_plan()
+    assert segs >= 2 and H.last_placement()[1] >= 2
+    assert np.array_equal(img, want)
+    H.close()
+
+
 def test_placement_arguments(pyborn):
     c = make_case(24, 24, 24, nt=10, nbound=4)
     H = hip_prop(pyborn, c)
