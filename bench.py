@@ -378,6 +378,9 @@ def main():
                     help="fix the launch shapes of the single-shot kernels: a JSON file or "
                          "string in the form of config.tuning of an earlier line (profiling "
                          "runs that must launch the shapes a full run chose)")
+    ap.add_argument("--placement", default="timed", choices=["timed", "off", "cycle"],
+                    help="where the work pair of the FP32 / narrow history sits (profiling "
+                         "runs compare \"off\": no gaps, the work buffers)")
     ap.add_argument("--step-pairs", type=int, default=0,
                     help="FP32 forward in two-step launches: 0 off, 1 when timed faster, 2 on")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -423,6 +426,7 @@ def main():
     if args.zchunk:
         P.set_zchunk(args.zchunk)
     P.set_step_pairs(args.step_pairs)
+    P.set_placement(args.placement)
     if args.tuning:
         text = open(args.tuning).read() if os.path.exists(args.tuning) else args.tuning
         fixed = json.loads(text)
@@ -485,6 +489,7 @@ def main():
     budget = int(args.budget_gb * 2 ** 30)
     elapsed, fwd_events, adj_events = run_leg("f32", args.steps, args.warmup, budget)
     segs, recomputed = P.last_checkpoint_plan()
+    moves = P.last_placement()
     # background steps the adjoint swept before its segments (0: the forward
     # kept the history or its checkpoint plan)
     swept = P.last_background_sweep()
@@ -556,6 +561,10 @@ def main():
             "adjoint_ms_per_shot": round(float(adj_ms), 2),
             "checkpoint_segments": segs,
             "recomputed_steps": recomputed,
+            # moves of the forward's r pair and of the adjoint state's pair
+            # between the gaps of the history, per shot
+            "placement": {"mode": args.placement, "forward_moves": moves[0],
+                          "adjoint_moves": moves[1]},
             "tuning": {k: v for k, v in tuning.items() if v["ms"] >= 0},
         },
         "roofline": {

@@ -1299,12 +1299,13 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   // Where the work pair sits (r, or with narrow storage s and r together),
   // by the window of the history level a step writes: level it of a kept
   // history, level it - klast in the last segment of a checkpoint plan;
-  // before klast (s in the work buffers) r stays in the work buffers.
+  // before klast (s in the work buffers) a window of its own, index ngaps.
   const int klast = ckS > 1 ? (ckS - 1) * ckL : nt;
   const bool place = ctx->place_mode && ctx->seg_gap && !use_pairs && (hist || ckS > 1);
+  const int wwork = hist ? 0 : 1;  // the window of steps that write no history level
   auto window_of = [&](int it) {
     const long long lv = hist ? it : it >= klast ? it - klast : -1;
-    return lv < 0 ? -1 : (int)(lv / ctx->seg_lpw);
+    return lv < 0 ? ctx->seg_ngaps : (int)(lv / ctx->seg_lpw);
   };
   std::vector<int> rplaces;
   if (place) {
@@ -1315,9 +1316,16 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     t.h = half_state(ctx, 0); t.rg = ctx->d_ring + RING_T0; t.hscale = ctx->d_ring + HS_T;
     const std::vector<uintptr_t> key = {
         (uintptr_t)ctx->d_seg, ctx->seg_win, (uintptr_t)scatter, (uintptr_t)n,
-        (uintptr_t)ctx->d_vel2, (uintptr_t)model, (uintptr_t)tu.shape, (uintptr_t)tu.zchunk};
-    CHK(place_plan(ctx, ctx->place_fwd, ctx->place_fwd_key, key, ctx->seg_ngaps, rplaces,
-                   [&](int w, int g, int i) {
+        (uintptr_t)ctx->d_vel2, (uintptr_t)model, (uintptr_t)tu.shape, (uintptr_t)tu.zchunk,
+        (uintptr_t)wwork, (uintptr_t)ctx->d_w[0]};
+    CHK(place_plan(ctx, ctx->place_fwd, ctx->place_fwd_key, key, ctx->seg_ngaps + wwork,
+                   rplaces, [&](int w, int g, int i) {
+      if (w == ctx->seg_ngaps) {  // s in the work buffers, no history level
+        t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.s_new = nullptr; t.hist = nullptr;
+        t.r0 = seg_gap_field(ctx, g, i % 2);
+        t.r1 = seg_gap_field(ctx, g, 1 - i % 2);
+        return born::launch_fwd_step(geo, tu, t, ctx->stream);
+      }
       // three levels of the window, one on per launch, as in a march
       const long long a = (long long)w * ctx->seg_lpw;
       const long long b = std::min(nlev, a + ctx->seg_lpw);
@@ -1411,8 +1419,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   for (; it < nt; it++) {
     const int id = it / jt, ii = it - id * jt;
     if (place) {
-      const int w = window_of(it);
-      const int want = w < 0 ? -1 : rplaces[w];
+      const int want = rplaces[window_of(it)];
       if (want != rplace) {
         if (half)
           CHK(move_pair(ctx, want, ctx->d_w[0], ctx->d_w[1], s_old, s_cur, 2 * Fb));
