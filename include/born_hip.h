@@ -201,21 +201,22 @@ int born_set_step_pairs(born_ctx *ctx, int mode);
  * last exact adjoint did */
 int born_last_step_pairs(const born_ctx *ctx);
 
-/* Placement of the scattered field's two levels (Born forward) and of the
- * adjoint state's two levels (exact adjoint) in HBM, for FP32 histories
- * longer than one window.  A step kernel's speed depends on where the
- * arrays it streams sit relative to one another (up to 13 % at 304^3, by
- * 32 GiB windows of history offset), so such a history is laid out in
- * windows at fixed offsets of its buffer, each a gap of two fields and
- * then levels, and the pair of levels moves (two field copies) from gap to
- * gap as the history levels a step touches change window.  mode 1
- * (default): each window takes the gap timed fastest for it (timed once
- * per grid, history buffer and the buffers of velocity and model; the work
- * buffers when the autotuner is off); 2: the work buffers and the gaps in turn,
+/* Placement in HBM of the two levels of the scattered field (Born forward;
+ * of s and r together with FP16 or 24-bit storage) and of the adjoint
+ * state (exact adjoint), for histories longer than one window.  A step
+ * kernel's speed depends on where the arrays it streams sit relative to
+ * one another (up to 20 % at 304^3, by 32 GiB windows of history offset),
+ * so such a history is laid out in windows at fixed offsets of its buffer,
+ * each a gap for two levels of the pair and then history levels, and the
+ * pair moves (two level copies) between the work buffers and the gaps as
+ * the history levels a step touches change window.  mode 1 (default): each
+ * window takes the place timed fastest for it (timed once per grid,
+ * history buffer and the buffers of velocity and model; the work buffers
+ * when the autotuner is off); 2: the work buffers and the gaps in turn,
  * window after window; 0: no gaps and no moves.  window_bytes: bytes per
- * window (0: 32 GiB; at least three fields).  Results do not depend on
- * either.  No reference counterpart: the reference holds no history on a
- * GPU. */
+ * window (0: 32 GiB; at least a gap and one level).  Results do not depend
+ * on either.  No reference counterpart: the reference holds no history on
+ * a GPU. */
 int born_set_placement(born_ctx *ctx, int mode, long long window_bytes);
 /* pair moves of the last Born forward and of the last exact adjoint (the
  * first move, from the work buffers, counts) */

@@ -113,23 +113,24 @@ struct born_ctx {
   int last_pairs = 0;
   int last_lam_pairs = 0;
 
-  // Placement of a two-field work pair: r of the forward, nu of the
-  // adjoint state.  A step's launch time depends on where its streams sit
-  // in HBM relative to one another (the FP32 forward at 304^3: 157-178 us
-  // for the same kernel and shape, by 32 GiB windows of history offset), so
-  // a history of FP32 levels longer than one window is laid out in windows
-  // of seg_win bytes at fixed offsets of d_seg, each a gap of two fields
-  // followed by seg_lpw levels, and the pair sits in the gap timed fastest
-  // for the window of the history levels a step touches.  place_mode 0
-  // keeps the plain layout (no gaps), 1 times the places, 2 cycles through
-  // them (tests).
+  // Placement of the work pair: the two levels of r of the FP32 forward (of
+  // s and r together with narrow storage) and of nu of the adjoint state.
+  // A step's launch time depends on where its streams sit in HBM relative
+  // to one another (at 304^3 the FP32 forward 157-178 us and the 24-bit
+  // forward 162-194 us for the same kernel and shape, by 32 GiB windows of
+  // history offset), so a history longer than one window is laid out in
+  // windows of seg_win bytes at fixed offsets of d_seg, each a gap for two
+  // levels of the pair followed by seg_lpw levels, and the pair sits in the
+  // work buffers or the gap timed fastest for the window of the history
+  // levels a step touches.  place_mode 0 keeps the plain layout (no gaps),
+  // 1 times the places, 2 cycles through them (tests).
   int place_mode = 1;
   size_t place_window = (size_t)32 << 30;  // history bytes per window
   size_t seg_gap = 0;     // bytes of a gap of d_seg (0: no gaps)
   size_t seg_win = 0;     // bytes per window
   long long seg_lpw = 0;  // levels per window
   int seg_ngaps = 0;
-  // the gap timed fastest per window (-1: not timed) and what it was timed
+  // the place timed fastest per window (-2: not timed) and what it was timed
   // for: the history buffer and window size, the grid, the other arrays the
   // step streams (their places matter too) and the launch shape
   std::vector<int> place_fwd, place_lam;
@@ -709,15 +710,15 @@ int move_pair(born_ctx *ctx, int to, float *w0, float *w1, float *&a, float *&b,
 
 // The place of the work pair in each of the nwin windows of the current
 // layout, into use[w] (-1: the work buffers, g >= 0: gap g).  place_mode 1:
-// the gap timed fastest over 4 launches of run(window, gap, i), i = 1, 2, ...
-// the launch's position in a march (after an untimed i = 0); a choice is
-// kept in `cache` and timed again only when `key` changes or the gap it
+// the place timed fastest over 2 launches of run(window, place, i), i = 1,
+// 2, ... the launch's position in a march (after an untimed i = 0); a choice
+// is kept in `cache` and timed again only when `key` changes or the gap it
 // names is past this layout's; without autotuning, the work buffers.
-// The differences it finds are 5-13 % and come in classes: the gaps that
+// The differences it finds are 5-20 % and come in classes: the gaps that
 // time within 2 % of each other on the first window timed give the same
 // time on every window (they sit in the same kind of HBM window), so that
-// window times every gap and the later ones one gap of each class, the
-// fastest of the class on the first.
+// window times every place and the later ones the work buffers and one gap
+// of each class, the fastest of the class on the first.
 // place_mode 2: window w takes place w mod (gaps + 1) - 1, every place in
 // turn.
 template <class F>
@@ -734,7 +735,7 @@ int place_plan(born_ctx *ctx, std::vector<int> &cache, std::vector<uintptr_t> &h
     cache.clear();
     held = key;
   }
-  if ((int)cache.size() < nwin) cache.resize(nwin, -1);
+  if ((int)cache.size() < nwin) cache.resize(nwin, -2);  // -2: not timed
   constexpr int reps = 2;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   auto time_gap = [&](int w, int g, float *ms) -> int {
@@ -750,11 +751,14 @@ int place_plan(born_ctx *ctx, std::vector<int> &cache, std::vector<uintptr_t> &h
     HIPCHK(hipEventElapsedTime(ms, e0, e1));
     return BORN_OK;
   };
-  std::vector<int> classes;  // one gap per class, after the first timed window
+  // the work buffers, then one gap per class (after the first timed window)
+  std::vector<int> classes;
   int rc = BORN_OK;
   for (int w = 0; w < nwin && rc == BORN_OK; w++) {
-    if (cache[w] < 0 || cache[w] >= ng) {
+    if (cache[w] < -1 || cache[w] >= ng) {
       if (classes.empty()) {
+        float tw = 0.f;
+        rc = time_gap(w, -1, &tw);
         std::vector<std::pair<float, int>> t(ng);
         for (int g = 0; g < ng && rc == BORN_OK; g++) {
           t[g].second = g;
@@ -762,10 +766,11 @@ int place_plan(born_ctx *ctx, std::vector<int> &cache, std::vector<uintptr_t> &h
         }
         if (rc != BORN_OK) break;
         std::sort(t.begin(), t.end());
+        classes.push_back(-1);
         classes.push_back(t[0].second);
         for (int k = 1; k < ng; k++)
           if (t[k].first > 1.02f * t[k - 1].first) classes.push_back(t[k].second);
-        cache[w] = t[0].second;
+        cache[w] = tw <= t[0].first ? -1 : t[0].second;
       } else {
         float best = 1e30f;
         for (int g : classes) {
@@ -1320,19 +1325,23 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
         (uintptr_t)wwork, (uintptr_t)ctx->d_w[0]};
     CHK(place_plan(ctx, ctx->place_fwd, ctx->place_fwd_key, key, ctx->seg_ngaps + wwork,
                    rplaces, [&](int w, int g, int i) {
+      // the pair's two levels at place g
+      float *p0 = g < 0 ? ctx->d_w[half ? 0 : 2] : seg_gap_field(ctx, g, 0);
+      float *p1 = g < 0 ? ctx->d_w[half ? 1 : 3] : seg_gap_field(ctx, g, 1);
+      if (i % 2) std::swap(p0, p1);
       if (w == ctx->seg_ngaps) {  // s in the work buffers, no history level
         t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.s_new = nullptr; t.hist = nullptr;
-        t.r0 = seg_gap_field(ctx, g, i % 2);
-        t.r1 = seg_gap_field(ctx, g, 1 - i % 2);
+        t.r0 = p0;
+        t.r1 = p1;
         return born::launch_fwd_step(geo, tu, t, ctx->stream);
       }
       // three levels of the window, one on per launch, as in a march
       const long long a = (long long)w * ctx->seg_lpw;
       const long long b = std::min(nlev, a + ctx->seg_lpw);
       const long long k = a + i % std::max(1LL, b - a - 2);
-      if (half) {  // s and r together in the gap, the scatter operand apart
-        t.s0 = seg_gap_field(ctx, g, i % 2);
-        t.s1 = seg_gap_field(ctx, g, 1 - i % 2);
+      if (half) {  // s and r together in the pair, the scatter operand apart
+        t.s0 = p0;
+        t.s1 = p1;
         t.s_new = nullptr;
         t.hist = seg_level(ctx, k);
         return born::launch_fwd_step(geo, tu, t, ctx->stream);
@@ -1346,8 +1355,8 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
         t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.s_new = nullptr;
         t.hist = seg_level(ctx, k);
       }
-      t.r0 = seg_gap_field(ctx, g, i % 2);
-      t.r1 = seg_gap_field(ctx, g, 1 - i % 2);
+      t.r0 = p0;
+      t.r1 = p1;
       return born::launch_fwd_step(geo, tu, t, ctx->stream); }));
   }
   for (int i = 0; i < 4; i++) CHK(zero(ctx, ctx->d_w[i], (long long)ctx->w_cap));
@@ -1779,8 +1788,10 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
       const long long a = (long long)w * ctx->seg_lpw;
       const long long b = std::min<long long>(L, a + ctx->seg_lpw);
       t.sig = seg_level(ctx, a + i % std::max(1LL, b - a));
-      t.u0 = seg_gap_field(ctx, g, i % 2);
-      t.u1 = seg_gap_field(ctx, g, 1 - i % 2);
+      float *u0 = g < 0 ? ctx->d_w[0] : seg_gap_field(ctx, g, 0);
+      float *u1 = g < 0 ? ctx->d_w[1] : seg_gap_field(ctx, g, 1);
+      t.u0 = i % 2 ? u1 : u0;
+      t.u1 = i % 2 ? u0 : u1;
       return born::launch_lam_step(geo, ltu, t, ctx->stream); }));
   }
   // pass 2: adjoint state, segment by segment in reverse
