@@ -207,7 +207,15 @@ struct FwdOp {
     if (pl.pm) sn = add_points(pts, pl.pm, c.x, c.y, sn, amp);
     float rn = leap(bb, v, lap[1], ctr[1], p[1]);
     float sig;
-    born_scatter<SCATTER>(bb, lap[0], sn, p[3], rn, sig);
+    if constexpr (kNarrow<T> && SCATTER == 0) {
+      // FP16 and 24-bit levels: the 1e-6 s m term in FP32.  FP32 levels
+      // round it in FP64 as the reference does (bitwise parity); the
+      // narrow levels' own rounding (2^-11, 2^-16) is far above FP32's.
+      rn = fma_r(mul_r(1e-6f, sn), p[3], rn);
+      sig = sn;
+    } else {
+      born_scatter<SCATTER>(bb, lap[0], sn, p[3], rn, sig);
+    }
     out[0] = sn;
     out[1] = rn;
     out[2] = sig;
