@@ -207,9 +207,10 @@ int born_last_step_pairs(const born_ctx *ctx);
  * kernel's speed depends on where the arrays it streams sit relative to
  * one another (up to 20 % at 304^3, by 32 GiB windows of history offset),
  * so such a history is laid out in windows at fixed offsets of its buffer,
- * each a gap for two levels of the pair and then history levels, and the
- * pair moves (two level copies) between the work buffers and the gaps as
- * the history levels a step touches change window.  mode 1 (default): each
+ * each a gap for two levels of the pair (the adjoint state's with the image
+ * accumulator) and then history levels, and the pair moves (two or three
+ * copies) between the work buffers and the gaps as the history levels a
+ * step touches change window.  mode 1 (default): each
  * window takes the place timed fastest for it (timed once per grid,
  * history buffer and the buffers of velocity and model; the work buffers
  * when the autotuner is off); 2: the work buffers and the gaps in turn,

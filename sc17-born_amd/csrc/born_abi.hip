@@ -85,6 +85,8 @@ struct born_ctx {
   size_t w_cap = 0;  // floats per work buffer
   float *d_w2[4] = {nullptr, nullptr, nullptr, nullptr};  // levels k+1, k+2 of pairs
   float *d_img = nullptr, *d_acc = nullptr, *d_p0 = nullptr, *d_p1 = nullptr;
+  float *d_vm = nullptr;  // velocity 2 and the forward's model interleaved per cell
+  size_t vm_cap = 0;
   float *d_tr = nullptr, *d_tr2 = nullptr;
   size_t tr_cap = 0;
   float *d_amp = nullptr;
@@ -120,9 +122,10 @@ struct born_ctx {
   // forward 162-194 us for the same kernel and shape, by 32 GiB windows of
   // history offset), so a history longer than one window is laid out in
   // windows of seg_win bytes at fixed offsets of d_seg, each a gap for two
-  // levels of the pair followed by seg_lpw levels, and the pair sits in the
-  // work buffers or the gap timed fastest for the window of the history
-  // levels a step touches.  place_mode 0 keeps the plain layout (no gaps),
+  // levels of the pair (and, for the adjoint state, the image accumulator)
+  // followed by seg_lpw levels, and the pair sits in the work buffers or
+  // the gap timed fastest for the window of the history levels a step
+  // touches.  place_mode 0 keeps the plain layout (no gaps),
   // 1 times the places, 2 cycles through them (tests).
   int place_mode = 1;
   size_t place_window = (size_t)32 << 30;  // history bytes per window
@@ -563,9 +566,10 @@ SegLayout seg_layout(const born_ctx *ctx, size_t L, size_t esz) {
   const size_t F = (size_t)ctx->n * esz;
   SegLayout s{0, 0, 0, 0, L * F};
   const size_t W = ctx->place_window;
-  // a gap holds two levels of the work pair: FP32 r levels, or the s and r
-  // levels that narrow storage keeps together
-  const size_t G = 2 * (esz == 4 ? F : 2 * F);
+  // a gap holds two levels of the forward's work pair (FP32 r levels, or
+  // the s and r levels that narrow storage keeps together) or the adjoint
+  // state's two levels and the FP32 image accumulator
+  const size_t G = esz == 4 ? 3 * F : 4 * F;
   if (ctx->place_mode == 0 || F == 0 || W < G + F) return s;
   const long long lpw = (long long)((W - G) / F);
   if ((long long)L <= lpw) return s;
@@ -586,9 +590,9 @@ float *seg_level(const born_ctx *ctx, long long k) {
   return (float *)((char *)ctx->d_seg + off);
 }
 
-// level i (0 or 1) of the work pair in gap g of the segment history
-float *seg_gap_field(const born_ctx *ctx, int g, int i) {
-  return (float *)((char *)ctx->d_seg + (size_t)g * ctx->seg_win + (size_t)i * ctx->seg_gap / 2);
+// the i-th field of `step` bytes in gap g of the segment history
+float *seg_gap_field(const born_ctx *ctx, int g, int i, size_t step) {
+  return (float *)((char *)ctx->d_seg + (size_t)g * ctx->seg_win + (size_t)i * step);
 }
 
 // history buffers: L levels of scatter operand + nck checkpoint levels, each
@@ -698,8 +702,8 @@ float *nth(float *base, long long k, long long n, size_t esz) {
 // work buffers w0, w1; g >= 0: gap g of the segment history): two copies on
 // the stream, none when the pair is there already.
 int move_pair(born_ctx *ctx, int to, float *w0, float *w1, float *&a, float *&b, size_t bytes) {
-  float *d0 = to < 0 ? w0 : seg_gap_field(ctx, to, 0);
-  float *d1 = to < 0 ? w1 : seg_gap_field(ctx, to, 1);
+  float *d0 = to < 0 ? w0 : seg_gap_field(ctx, to, 0, bytes);
+  float *d1 = to < 0 ? w1 : seg_gap_field(ctx, to, 1, bytes);
   if ((a == d0 && b == d1) || (a == d1 && b == d0)) return BORN_OK;
   HIPCHK(hipMemcpyAsync(d0, a, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(d1, b, bytes, hipMemcpyDeviceToDevice, ctx->stream));
@@ -723,8 +727,11 @@ int move_pair(born_ctx *ctx, int to, float *w0, float *w1, float *&a, float *&b,
 // turn.
 template <class F>
 int place_plan(born_ctx *ctx, std::vector<int> &cache, std::vector<uintptr_t> &held,
-               const std::vector<uintptr_t> &key, int nwin, std::vector<int> &use, F run) {
-  const int ng = ctx->seg_ngaps;
+               const std::vector<uintptr_t> &key, int nwin, std::vector<int> &use, F run,
+               int ncand = -1) {
+  // places: -1 the work buffers, 0 .. ng - 1 the gaps (and, where the
+  // caller has variants of a gap, ng .. ncand - 1)
+  const int ng = ncand < 0 ? ctx->seg_ngaps : ncand;
   use.assign(nwin, -1);
   if (ctx->place_mode == 2) {
     for (int w = 0; w < nwin; w++) use[w] = w % (ng + 1) - 1;
@@ -1004,7 +1011,7 @@ int born_destroy(born_ctx *ctx) {
                   ctx->d_w2[1], ctx->d_w2[2], ctx->d_w2[3], ctx->d_img,
                   ctx->d_acc,   ctx->d_p0,    ctx->d_p1,   ctx->d_tr,
                   ctx->d_tr2,   ctx->d_amp,   ctx->d_seg,  ctx->d_ckpt,
-                  ctx->d_ring,  ctx->d_hscale, ctx->d_ckring};
+                  ctx->d_ring,  ctx->d_hscale, ctx->d_ckring, ctx->d_vm};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   (void)hipStreamDestroy(ctx->stream);
@@ -1047,6 +1054,8 @@ int born_create_space(born_ctx *ctx, float d1, float d2, float d3, float bc_a,
     for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &ctx->d_w2[i], 0));
     CHK(dev_alloc(ctx, &ctx->d_img, 0));
     CHK(dev_alloc(ctx, &ctx->d_acc, 0));
+    CHK(dev_alloc(ctx, &ctx->d_vm, 0));
+    ctx->vm_cap = 0;
     CHK(dev_alloc(ctx, &ctx->d_p0, 0));
     CHK(dev_alloc(ctx, &ctx->d_p1, 0));
     // The history and checkpoint buffers stay: freeing a buffer of hundreds
@@ -1256,13 +1265,16 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   ctx->jt_state = jt;
   const Geometry geo = geometry(ctx);
   const int kind = fwd_kind(ctx->storage, scatter);
+  // velocity 2 and the model as one stream (8 bytes per cell)
+  CHK(ensure(ctx, &ctx->d_vm, &ctx->vm_cap, 2 * (size_t)n));
+  HIPCHK(born::launch_interleave(ctx->d_vm, ctx->d_vel2, model, n, ctx->stream));
   {  // tune on the work fields (zeroed below) and scratch scales, no traces written
     CHK(ring_reset(ctx, RING_T0));
     CHK(ring_reset(ctx, RING_T1));
     HIPCHK(born::launch_fill(ctx->d_ring + MAX_T, 2, 0.f, ctx->stream));
     born::FwdStep t{};
     t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.r1 = ctx->d_w[3]; t.r0 = ctx->d_w[2];
-    t.vel = ctx->d_vel2; t.model = model; t.src = no_points(); t.rec = recmap(ctx);
+    t.vel = ctx->d_vel2; t.model = model; t.vm = ctx->d_vm; t.src = no_points(); t.rec = recmap(ctx);
     t.tr = traces; t.tab = ctx->d_tab_d; t.extract = 0; t.scatter = scatter;
     t.h = half_state(ctx, 0);
     t.rg = ctx->d_ring + RING_T0; t.hscale = ctx->d_ring + HS_T;
@@ -1316,7 +1328,8 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   if (place) {
     const long long nlev = hist ? nt : ckL;
     born::FwdStep t{};  // a step without sources or traces
-    t.vel = ctx->d_vel2; t.model = model; t.src = no_points(); t.rec = recmap(ctx);
+    t.vel = ctx->d_vel2; t.model = model; t.vm = ctx->d_vm; t.src = no_points();
+    t.rec = recmap(ctx);
     t.tr = traces; t.tab = ctx->d_tab_d; t.extract = 0; t.scatter = scatter;
     t.h = half_state(ctx, 0); t.rg = ctx->d_ring + RING_T0; t.hscale = ctx->d_ring + HS_T;
     const std::vector<uintptr_t> key = {
@@ -1326,8 +1339,9 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     CHK(place_plan(ctx, ctx->place_fwd, ctx->place_fwd_key, key, ctx->seg_ngaps + wwork,
                    rplaces, [&](int w, int g, int i) {
       // the pair's two levels at place g
-      float *p0 = g < 0 ? ctx->d_w[half ? 0 : 2] : seg_gap_field(ctx, g, 0);
-      float *p1 = g < 0 ? ctx->d_w[half ? 1 : 3] : seg_gap_field(ctx, g, 1);
+      const size_t pb = (half ? 2 : 1) * (size_t)n * esz;  // a level of the pair
+      float *p0 = g < 0 ? ctx->d_w[half ? 0 : 2] : seg_gap_field(ctx, g, 0, pb);
+      float *p1 = g < 0 ? ctx->d_w[half ? 1 : 3] : seg_gap_field(ctx, g, 1, pb);
       if (i % 2) std::swap(p0, p1);
       if (w == ctx->seg_ngaps) {  // s in the work buffers, no history level
         t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.s_new = nullptr; t.hist = nullptr;
@@ -1406,6 +1420,7 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   born::FwdStep a{};
   a.vel = ctx->d_vel2;
   a.model = model;
+  a.vm = ctx->d_vm;
   a.src = points(ctx);
   a.rec = recmap(ctx);
   a.tr = traces;
@@ -1782,17 +1797,21 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
     const std::vector<uintptr_t> key = {
         (uintptr_t)ctx->d_seg, ctx->seg_win, (uintptr_t)n, (uintptr_t)ctx->d_vel2,
         (uintptr_t)ctx->d_acc, (uintptr_t)ltu.shape, (uintptr_t)ltu.zchunk};
+    // places: gap g < ngaps holds nu, the accumulator stays in its buffer;
+    // gap g - ngaps holds both for g >= ngaps
     CHK(place_plan(ctx, ctx->place_lam, ctx->place_lam_key, key, ctx->seg_ngaps, uplaces,
                    [&](int w, int g, int i) {
       // the sigma levels of the window, one on per launch
       const long long a = (long long)w * ctx->seg_lpw;
       const long long b = std::min<long long>(L, a + ctx->seg_lpw);
       t.sig = seg_level(ctx, a + i % std::max(1LL, b - a));
-      float *u0 = g < 0 ? ctx->d_w[0] : seg_gap_field(ctx, g, 0);
-      float *u1 = g < 0 ? ctx->d_w[1] : seg_gap_field(ctx, g, 1);
+      const int gu = g < ctx->seg_ngaps ? g : g - ctx->seg_ngaps;
+      float *u0 = g < 0 ? ctx->d_w[0] : seg_gap_field(ctx, gu, 0, Fb);
+      float *u1 = g < 0 ? ctx->d_w[1] : seg_gap_field(ctx, gu, 1, Fb);
+      t.acc = g < ctx->seg_ngaps ? ctx->d_acc : seg_gap_field(ctx, gu, 2, Fb);
       t.u0 = i % 2 ? u1 : u0;
       t.u1 = i % 2 ? u0 : u1;
-      return born::launch_lam_step(geo, ltu, t, ctx->stream); }));
+      return born::launch_lam_step(geo, ltu, t, ctx->stream); }, 2 * ctx->seg_ngaps));
   }
   // pass 2: adjoint state, segment by segment in reverse
   float *u_old = ctx->d_w[0], *u_cur = ctx->d_w[1];
@@ -1807,7 +1826,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
   }
   born::LamStep lm{};
   lm.vel = ctx->d_vel2;
-  lm.acc = ctx->d_acc;
+  float *acc = ctx->d_acc;  // the image accumulator, moved with nu
   lm.rec = recmap(ctx);
   lm.tr = traces;
   lm.ring = ring(ctx, RING_U);
@@ -1884,13 +1903,20 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
       if (place) {
         const int want = uplaces[(int)((k - k0) / ctx->seg_lpw)];
         if (want != uplace) {
-          CHK(move_pair(ctx, want, ctx->d_w[0], ctx->d_w[1], ub[0], ub[1], Fb));
+          const int gu = want < ctx->seg_ngaps ? want : want - ctx->seg_ngaps;
+          CHK(move_pair(ctx, gu, ctx->d_w[0], ctx->d_w[1], ub[0], ub[1], Fb));
+          float *to = want < ctx->seg_ngaps ? ctx->d_acc : seg_gap_field(ctx, gu, 2, Fb);
+          if (to != acc)
+            HIPCHK(hipMemcpyAsync(to, acc, (size_t)n * sizeof(float),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+          acc = to;
           uplace = want;
           umoves++;
         }
       }
       lm.u1 = ub[1];
       lm.u0 = ub[0];
+      lm.acc = acc;
       lm.sig = seg_level(ctx, k - k0);
       lm.hscale = half ? ctx->d_hscale + (k - k0) : nullptr;
       lm.tab = ctx->d_tab_d + (long long)ii * 8;
@@ -1902,7 +1928,7 @@ int born_dev_adjoint_exact(born_ctx *ctx, int jt, int nt, int scatter,
       k--;
     }
   }
-  HIPCHK(born::launch_adj_finish(geo, img, ctx->d_acc, ctx->d_vel2,
+  HIPCHK(born::launch_adj_finish(geo, img, acc, ctx->d_vel2,
                                  scatter == 0 ? 1e-6f : 1.f, ctx->stream));
   ctx->last_segments = S;
   ctx->last_recomputed = recomputed;

@@ -182,6 +182,23 @@ hipError_t launch_batch_lam_step(const Taps &taps, const Tune &tu, const BatchLa
                     [&](auto t) { return batch_lam_step<decltype(t)>(taps, tu, b, st); });
 }
 
+// dst[2i] = a[i], dst[2i+1] = b[i]
+__global__ void interleave_kernel(float2 *dst, const float *a, const float *b, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = make_float2(a[i], b[i]);
+}
+
+hipError_t launch_interleave(float *dst, const float *a, const float *b, long long n,
+                             hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(interleave_kernel, dim3((unsigned)nb), dim3(256), 0, st, (float2 *)dst, a,
+                     b, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(float *p, long long n, float v, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   long long nb = (n + 255) / 256;

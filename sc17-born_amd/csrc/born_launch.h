@@ -50,6 +50,9 @@ struct FwdStep {
   float *s0, *r0;
   float *s_new;  // where the new s level goes (null: s0, in place)
   const float *vel, *model;
+  // vel and model interleaved cell by cell (2n floats): the single-shot
+  // forward reads them as one stream
+  const float *vm;
   float *hist;
   SrcPts src;
   const float *amp;
@@ -228,6 +231,9 @@ hipError_t launch_srec_step(const Geometry &geo, const Tune &tu, const SrecStep 
 hipError_t launch_lam_step(const Geometry &geo, const Tune &tu, const LamStep &a,
                            hipStream_t st);
 hipError_t launch_fill(float *p, long long n, float v, hipStream_t st);
+// dst (2n floats) = a and b interleaved cell by cell
+hipError_t launch_interleave(float *dst, const float *a, const float *b, long long n,
+                             hipStream_t st);
 hipError_t launch_src_amp(float *amp, int nsteps, int npts, const float *src, int ntsrc,
                           const float *tab, int nsinc, int jt, int ntab, int t_off,
                           float scale, hipStream_t st, int stride = -1);

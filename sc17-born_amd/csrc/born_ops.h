@@ -99,6 +99,9 @@ struct FwdOp {
   PairOf<T> *srn;
   const float *__restrict__ vel;
   const float *__restrict__ model;
+  // NT (the single-shot forward): vel and model interleaved per cell, one
+  // load and one stream for both
+  const float2 *__restrict__ vm;
   T *__restrict__ hist;  // optional scatter-operand history
   Points pts;
   const float *__restrict__ amp;  // source amplitudes of this step [npts]
@@ -182,8 +185,20 @@ struct FwdOp {
       ld_vec<VX>(r0, o, b.io[1], p[1], nlev);
     }
     if constexpr (NT) {
-      ld_stream<VX>(vel, o, 1.f, p[2]);
-      ld_stream<VX>(model, o, 1.f, p[3]);
+      const char *q = (const char *)vm + (uint64_t)o * 8u;
+      if constexpr (VX == 1) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const f2 w = __builtin_nontemporal_load((const f2 *)q);
+        p[2][0] = w.x;
+        p[3][0] = w.y;
+      } else {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const f4 w = __builtin_nontemporal_load((const f4 *)q);
+        p[2][0] = w.x;
+        p[3][0] = w.y;
+        p[2][1] = w.z;
+        p[3][1] = w.w;
+      }
     } else {
       ld_vec<VX>(vel, o, 1.f, p[2]);
       ld_vec<VX>(model, o, 1.f, p[3]);
@@ -758,7 +773,8 @@ hipError_t fwd_step(const Geometry &geo, const Tune &tu, const FwdStep &a, hipSt
   op.s_new = as<T>(a.s_new ? a.s_new : a.s0);
   op.sr1 = (const PairOf<T> *)a.s1; op.sr0 = (const PairOf<T> *)a.s0;
   op.srn = (PairOf<T> *)(a.s_new ? a.s_new : a.s0);
-  op.vel = a.vel; op.model = a.model; op.hist = as<T>(a.hist);
+  op.vel = a.vel; op.model = a.model; op.vm = (const float2 *)a.vm; op.hist = as<T>(a.hist);
+  if (!a.vm) return hipErrorInvalidValue;
   op.pts = to_points(a.src); op.amp = a.amp;
   op.rm = a.rec; op.tr = a.tr; op.tab = a.tab; op.id = a.id;
   op.extract = a.extract; op.nx = geo.nx; op.ny = geo.ny;

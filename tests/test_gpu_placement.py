@@ -40,12 +40,13 @@ def test_pair_placement_leaves_results_unchanged(pyborn, scatter):
     want_tr, want = _pair(H, c, torch, model, w, scatter)
     assert H.last_placement() == (0, 0)
     assert np.abs(want).max() > 0 and np.abs(want_tr).max() > 0
-    # windows of 8 fields: a gap of two, then 6 levels; 12 windows.  Window
-    # w takes place w - 1 (the work buffers first), so the pair moves at
-    # every window but the forward's first
+    # windows of 8 fields: a gap of three (two levels and the adjoint's image
+    # accumulator), then 5 levels; 14 windows.  Window w takes place w - 1
+    # (the work buffers first), so the pair moves at every window but the
+    # forward's first
     H.set_placement("cycle", 8 * F)
     tr, img = _pair(H, c, torch, model, w, scatter)
-    nwin = -(-c.nt // 6)
+    nwin = -(-c.nt // 5)
     assert H.last_placement() == (nwin - 1, nwin)
     assert np.array_equal(tr, want_tr) and np.array_equal(img, want)
     # the timed choice (whatever it picks)
@@ -54,7 +55,7 @@ def test_pair_placement_leaves_results_unchanged(pyborn, scatter):
     assert np.array_equal(tr, want_tr) and np.array_equal(img, want)
     # a checkpoint plan over a gapped segment history: the forward's last
     # segment and the adjoint's recomputed ones
-    H.set_placement("cycle", 4 * F)
+    H.set_placement("cycle", 6 * F)
     H.set_checkpoint_budget(40 * F)
     tr, img = _pair(H, c, torch, model, w, scatter)
     segs, _ = H.last_checkpoint_plan()
