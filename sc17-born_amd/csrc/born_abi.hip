@@ -484,6 +484,9 @@ int ensure_work(born_ctx *ctx) {
     for (int i = 0; i < 4; i++) CHK(dev_alloc(ctx, &ctx->d_w[i], need));
     ctx->w_cap = need;
   }
+  // the forward's interleaved velocity and model, allocated with the work
+  // buffers so that it exists before any history is sized to the free memory
+  CHK(ensure(ctx, &ctx->d_vm, &ctx->vm_cap, 2 * (size_t)ctx->n));
   return BORN_OK;
 }
 
@@ -1265,8 +1268,8 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
   ctx->jt_state = jt;
   const Geometry geo = geometry(ctx);
   const int kind = fwd_kind(ctx->storage, scatter);
-  // velocity 2 and the model as one stream (8 bytes per cell)
-  CHK(ensure(ctx, &ctx->d_vm, &ctx->vm_cap, 2 * (size_t)n));
+  // velocity 2 and the model as one stream (8 bytes per cell; the buffer
+  // comes with the work buffers)
   HIPCHK(born::launch_interleave(ctx->d_vm, ctx->d_vel2, model, n, ctx->stream));
   {  // tune on the work fields (zeroed below) and scratch scales, no traces written
     CHK(ring_reset(ctx, RING_T0));
