@@ -1277,7 +1277,8 @@ int born_dev_forward(born_ctx *ctx, int jt, int nt, int scatter, const float *mo
     HIPCHK(born::launch_fill(ctx->d_ring + MAX_T, 2, 0.f, ctx->stream));
     born::FwdStep t{};
     t.s1 = ctx->d_w[1]; t.s0 = ctx->d_w[0]; t.r1 = ctx->d_w[3]; t.r0 = ctx->d_w[2];
-    t.vel = ctx->d_vel2; t.model = model; t.vm = ctx->d_vm; t.src = no_points(); t.rec = recmap(ctx);
+    t.vel = ctx->d_vel2; t.model = model; t.vm = ctx->d_vm;
+    t.src = no_points(); t.rec = recmap(ctx);
     t.tr = traces; t.tab = ctx->d_tab_d; t.extract = 0; t.scatter = scatter;
     t.h = half_state(ctx, 0);
     t.rg = ctx->d_ring + RING_T0; t.hscale = ctx->d_ring + HS_T;
