@@ -554,10 +554,9 @@ size_t history_budget(born_ctx *ctx) {
   return fr > reserve ? fr - reserve : 0;
 }
 
-// Layout of a segment history of L levels of esz bytes: FP32 levels of a
-// history longer than one window are laid out in windows of W bytes at
-// offsets 0, W, 2W, ...: a gap of two fields (the places of the work pair,
-// see born_ctx), then lpw levels.
+// Layout of a segment history of L levels of esz bytes: a history longer
+// than one window is laid out in windows of W bytes at offsets 0, W, 2W,
+// ...: a gap (a place of the work pair, see born_ctx), then lpw levels.
 struct SegLayout {
   size_t gap;     // bytes of a gap, 0: no gaps
   size_t win;     // bytes per window
